@@ -1,0 +1,781 @@
+/*
+ * gabe_api.hip -- the C ABI declared in include/gabe.h: device state of one
+ * lattice, rocFFT plans and the kernel sequence of a step.
+ */
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gabe.h"
+#include "gabe_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(GABE_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define FFTCHK(expr)                                                                        \
+    do {                                                                                    \
+        rocfft_status s_ = (expr);                                                          \
+        if (s_ != rocfft_status_success)                                                    \
+            return fail(GABE_ERR_FFT, "%s failed: rocfft status %d", #expr, (int)s_);      \
+    } while (0)
+
+#define CHK(expr)                        \
+    do {                                 \
+        int r_ = (expr);                 \
+        if (r_ != GABE_OK) return r_;    \
+    } while (0)
+
+bool g_rocfft_ready = false;
+
+/* Neumaier-compensated sum in index order */
+double csum(const double *x, long n)
+{
+    double s = 0., c = 0.;
+    for (long i = 0; i < n; i++) {
+        double t = s + x[i];
+        if (std::fabs(s) >= std::fabs(x[i])) c += (s - t) + x[i];
+        else c += (x[i] - t) + s;
+        s = t;
+    }
+    return s + c;
+}
+
+} // namespace
+
+struct FftPlan {
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    size_t work = 0;
+};
+
+struct gabe_ctx {
+    gabe_params p{};
+    int N = 0, nf = 0;
+    long N2 = 0, N3 = 0, M = 0;
+    double dx = 0.;
+    bool fp32 = false, gw = false, alloc_gw = false;
+    size_t esz = 8;              /* bytes per stored real */
+    hipStream_t stream = nullptr;
+    void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
+    void *vbuf[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *stress = nullptr;      /* [ns_arrays][N3] real */
+    void *shat = nullptr;        /* [ns_arrays][M] complex */
+    void *h = nullptr, *l = nullptr; /* [6][M] complex */
+    double *partials = nullptr;
+    int nblk = 0, ci = 0;
+    dim3 grid;
+    DevScalars *sc = nullptr;
+    StepScalars *ss = nullptr;
+    double *hist = nullptr;
+    int hist_cap = 0;
+    double *spec_out = nullptr;
+    FftPlan fft_stress, fft_spec, fft_c2r;
+    void *fft_work = nullptr;
+    size_t fft_work_size = 0;
+    gabe_model model{};
+};
+
+namespace {
+
+int ensure_rocfft()
+{
+    if (!g_rocfft_ready) {
+        FFTCHK(rocfft_setup());
+        g_rocfft_ready = true;
+    }
+    return GABE_OK;
+}
+
+int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch)
+{
+    CHK(ensure_rocfft());
+    size_t len[3] = {(size_t)c->N, (size_t)c->N, (size_t)c->N};
+    FFTCHK(rocfft_plan_create(&P.plan, rocfft_placement_notinplace, type,
+                              dbl ? rocfft_precision_double : rocfft_precision_single, 3, len, batch, nullptr));
+    FFTCHK(rocfft_plan_get_work_buffer_size(P.plan, &P.work));
+    FFTCHK(rocfft_execution_info_create(&P.info));
+    FFTCHK(rocfft_execution_info_set_stream(P.info, c->stream));
+    if (P.work > c->fft_work_size) {
+        if (c->fft_work) HIPCHK(hipFree(c->fft_work));
+        c->fft_work = nullptr;
+        HIPCHK(hipMalloc(&c->fft_work, P.work));
+        c->fft_work_size = P.work;
+    }
+    return GABE_OK;
+}
+
+int run_plan(gabe_ctx *c, FftPlan &P, void *in, void *out)
+{
+    if (P.work) FFTCHK(rocfft_execution_info_set_work_buffer(P.info, c->fft_work, P.work));
+    void *ib[1] = {in}, *ob[1] = {out};
+    FFTCHK(rocfft_execute(P.plan, ib, ob, P.info));
+    return GABE_OK;
+}
+
+void destroy_plan(FftPlan &P)
+{
+    if (P.plan) rocfft_plan_destroy(P.plan);
+    if (P.info) rocfft_execution_info_destroy(P.info);
+    P.plan = nullptr;
+    P.info = nullptr;
+}
+
+template <typename T>
+SweepArgs<T> sweep_args(gabe_ctx *c)
+{
+    SweepArgs<T> a{};
+    for (int f = 0; f < c->nf; f++) {
+        a.f[f] = (const T *)c->fbuf[c->cur] + (long)f * c->N3;
+        a.v[f] = (const T *)c->vbuf[c->cur] + (long)f * c->N3;
+        a.fo[f] = (T *)c->fbuf[1 - c->cur] + (long)f * c->N3;
+        a.vo[f] = (T *)c->vbuf[1 - c->cur] + (long)f * c->N3;
+    }
+    a.stress = (T *)c->stress;
+    a.partials = c->partials;
+    a.sc = c->sc;
+    a.ss = c->ss;
+    a.model = c->model;
+    a.N = c->N;
+    a.ci = c->ci;
+    a.nblk = c->nblk;
+    a.N2 = c->N2;
+    a.N3 = c->N3;
+    a.dt = c->p.dt;
+    a.inv_dx2 = 1. / (c->dx * c->dx);
+    return a;
+}
+
+int launch_sweep_ctx(gabe_ctx *c, int mode, bool gw)
+{
+    if (c->fp32) launch_sweep<float>(mode, c->nf, gw, sweep_args<float>(c), c->grid, c->stream);
+    else launch_sweep<double>(mode, c->nf, gw, sweep_args<double>(c), c->grid, c->stream);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+int launch_finalize_ctx(gabe_ctx *c, int mode, int refresh_adot)
+{
+    FinArgs F{};
+    F.partials = c->partials;
+    F.nblk = c->nblk;
+    F.nf = c->nf;
+    F.mode = mode;
+    F.exp_type = c->p.expansion_type;
+    F.split = c->fp32 ? 1 : 0;
+    F.refresh_adot = refresh_adot;
+    F.N3 = (double)c->N3;
+    F.dt = c->p.dt;
+    F.sc = c->sc;
+    F.ss = c->ss;
+    F.hist = c->hist;
+    F.hist_cap = c->hist_cap;
+    launch_finalize(F, c->stream);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+int launch_gw_ctx(gabe_ctx *c)
+{
+    const double dx3 = c->dx * c->dx * c->dx;
+    if (c->fp32) {
+        GwArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, c->N, c->M, c->p.L, c->p.dt, dx3};
+        launch_gw<float>(g, c->stream);
+    } else {
+        GwArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss, c->N, c->M, c->p.L, c->p.dt, dx3};
+        launch_gw<double>(g, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+/* the kernels of one step; ev (optional) receives 2 events per kernel class */
+int one_step(gabe_ctx *c, hipEvent_t *ev)
+{
+    auto rec = [&](int k) -> int {
+        if (ev) HIPCHK(hipEventRecord(ev[k], c->stream));
+        return GABE_OK;
+    };
+    CHK(rec(0));
+    CHK(launch_sweep_ctx(c, SW_A, c->gw));
+    CHK(rec(1));
+    CHK(rec(2));
+    CHK(launch_finalize_ctx(c, SW_A, 0));
+    CHK(rec(3));
+    CHK(rec(4));
+    if (c->gw) CHK(run_plan(c, c->fft_stress, c->stress, c->shat));
+    CHK(rec(5));
+    CHK(rec(6));
+    if (c->gw) CHK(launch_gw_ctx(c));
+    CHK(rec(7));
+    CHK(rec(8));
+    CHK(launch_sweep_ctx(c, SW_B, false));
+    CHK(rec(9));
+    c->cur = 1 - c->cur;
+    return GABE_OK;
+}
+
+int read_scalars(gabe_ctx *c, DevScalars &d)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&d, c->sc, sizeof d, hipMemcpyDeviceToHost));
+    return GABE_OK;
+}
+
+int write_scalars(gabe_ctx *c, const DevScalars &d)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->sc, &d, sizeof d, hipMemcpyHostToDevice));
+    return GABE_OK;
+}
+
+/* energies (and adot[0] if stale) of the current state */
+int refresh_e0(gabe_ctx *c)
+{
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    if (d.e0_valid && d.adot0_valid) return GABE_OK;
+    CHK(launch_sweep_ctx(c, SW_ENERGY, false));
+    CHK(launch_finalize_ctx(c, SW_ENERGY, 1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GABE_OK;
+}
+
+int check_params(const gabe_params *p)
+{
+    if (!p) return fail(GABE_ERR_ARG, "params is NULL");
+    if (p->N < 4 || (p->N & 1)) return fail(GABE_ERR_ARG, "N must be even and >= 4 (got %d)", p->N);
+    if (p->nflds < 1 || p->nflds > GABE_MAX_FLDS) return fail(GABE_ERR_ARG, "nflds must be 1 or 2 (got %d)", p->nflds);
+    if (!(p->L > 0.) || !(p->dt > 0.)) return fail(GABE_ERR_ARG, "L and dt must be positive");
+    if (p->expansion_type < 0 || p->expansion_type > 2) return fail(GABE_ERR_ARG, "expansion_type must be 0, 1 or 2");
+    if (p->precision != GABE_FP64 && p->precision != GABE_FP32) return fail(GABE_ERR_ARG, "precision must be GABE_FP64 or GABE_FP32");
+    if ((long)p->N * p->N * p->N > (1L << 31)) return fail(GABE_ERR_ARG, "N^3 beyond 2^31 is not supported");
+    return GABE_OK;
+}
+
+int create_impl(const gabe_params *params, gabe_ctx *c)
+{
+    c->p = *params;
+    c->N = params->N;
+    c->nf = params->nflds;
+    c->N2 = (long)c->N * c->N;
+    c->N3 = c->N2 * c->N;
+    c->M = c->N2 * (c->N / 2 + 1);
+    c->dx = params->L / (double)c->N;
+    c->fp32 = params->precision == GABE_FP32;
+    c->esz = c->fp32 ? 4 : 8;
+    c->alloc_gw = params->calc_gws != 0;
+    c->gw = params->calc_gws && params->expansion_type == 1; /* Q15: only type 1 evolves GWs */
+    c->model = gabe_model{params->m2_phi, params->lambda, params->g2, params->m2_chi};
+
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const size_t fbytes = c->esz * c->nf * c->N3;
+    for (int s = 0; s < 2; s++) {
+        HIPCHK(hipMalloc(&c->fbuf[s], fbytes));
+        HIPCHK(hipMalloc(&c->vbuf[s], fbytes));
+        HIPCHK(hipMemset(c->fbuf[s], 0, fbytes));
+        HIPCHK(hipMemset(c->vbuf[s], 0, fbytes));
+    }
+    const int narr = c->alloc_gw ? 12 : c->nf;
+    HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
+    HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * c->M));
+    if (c->alloc_gw) {
+        HIPCHK(hipMalloc(&c->h, 2 * c->esz * 6 * c->M));
+        HIPCHK(hipMalloc(&c->l, 2 * c->esz * 6 * c->M));
+        HIPCHK(hipMemset(c->h, 0, 2 * c->esz * 6 * c->M));
+        HIPCHK(hipMemset(c->l, 0, 2 * c->esz * 6 * c->M));
+    }
+
+    /* sweep tiling: one thread per plane column, ci planes per workgroup */
+    const int bx = (int)((c->N2 + GABE_BLOCK - 1) / GABE_BLOCK);
+    int ci = params->planes_per_block;
+    if (ci <= 0) {
+        const int by_target = std::max(1, 2048 / bx);
+        ci = (c->N + by_target - 1) / by_target;
+    }
+    ci = std::min(std::max(ci, 1), c->N);
+    const int by = (c->N + ci - 1) / ci;
+    c->ci = ci;
+    c->grid = dim3(bx, by);
+    c->nblk = bx * by;
+    HIPCHK(hipMalloc(&c->partials, sizeof(double) * GABE_MAXSUMS * std::max(c->nblk, 4096)));
+
+    HIPCHK(hipMalloc(&c->sc, sizeof(DevScalars)));
+    HIPCHK(hipMalloc(&c->ss, sizeof(StepScalars)));
+    DevScalars d{};
+    d.a[0] = d.a[1] = 1.;
+    HIPCHK(hipMemcpy(c->sc, &d, sizeof d, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->ss, 0, sizeof(StepScalars)));
+    c->hist_cap = 1 << 16;
+    HIPCHK(hipMalloc(&c->hist, sizeof(double) * GABE_HIST_COLS * c->hist_cap));
+    HIPCHK(hipMemset(c->hist, 0, sizeof(double) * GABE_HIST_COLS * c->hist_cap));
+    const int nb = gabe_numbins(c->N);
+    HIPCHK(hipMalloc(&c->spec_out, sizeof(double) * nb * (GABE_MAXF + 2)));
+
+    if (c->alloc_gw) CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12));
+    CHK(make_plan(c, c->fft_spec, rocfft_transform_type_real_forward, !c->fp32, c->nf));
+    return GABE_OK;
+}
+
+void destroy_impl(gabe_ctx *c)
+{
+    destroy_plan(c->fft_stress);
+    destroy_plan(c->fft_spec);
+    destroy_plan(c->fft_c2r);
+    void *bufs[] = {c->fbuf[0], c->fbuf[1], c->vbuf[0], c->vbuf[1], c->stress, c->shat, c->h, c->l,
+                    c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+/* host [nf][N3] doubles -> device storage of slot cur (split into mean + fluctuation for FP32) */
+int upload_fields(gabe_ctx *c, void *dst, const double *src, double *means)
+{
+    const long n3 = c->N3;
+    if (c->fp32) {
+        std::vector<float> tmp(n3);
+        for (int f = 0; f < c->nf; f++) {
+            const double *x = src + (long)f * n3;
+            const double mean = csum(x, n3) / (double)n3;
+            for (long n = 0; n < n3; n++) tmp[n] = (float)(x[n] - mean);
+            HIPCHK(hipMemcpy((float *)dst + (long)f * n3, tmp.data(), sizeof(float) * n3, hipMemcpyHostToDevice));
+            means[f] = mean;
+        }
+    } else {
+        HIPCHK(hipMemcpy(dst, src, sizeof(double) * c->nf * n3, hipMemcpyHostToDevice));
+        for (int f = 0; f < c->nf; f++) means[f] = 0.;
+    }
+    return GABE_OK;
+}
+
+int download_fields(gabe_ctx *c, const void *src, double *dst, const double *means)
+{
+    const long n3 = c->N3;
+    if (c->fp32) {
+        std::vector<float> tmp(n3);
+        for (int f = 0; f < c->nf; f++) {
+            HIPCHK(hipMemcpy(tmp.data(), (const float *)src + (long)f * n3, sizeof(float) * n3, hipMemcpyDeviceToHost));
+            for (long n = 0; n < n3; n++) dst[(long)f * n3 + n] = means[f] + (double)tmp[n];
+        }
+    } else {
+        HIPCHK(hipMemcpy(dst, src, sizeof(double) * c->nf * n3, hipMemcpyDeviceToHost));
+    }
+    return GABE_OK;
+}
+
+} // namespace
+
+/* ====================================================================== ABI */
+extern "C" {
+
+const char *gabe_last_error(void) { return g_err.c_str(); }
+
+int gabe_numbins(int N) { return (int)(std::sqrt(3.) * N / 2 + 1); }
+
+int gabe_create(const gabe_params *params, gabe_ctx **out)
+{
+    if (!out) return fail(GABE_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    CHK(check_params(params));
+    gabe_ctx *c = new gabe_ctx;
+    int r = create_impl(params, c);
+    if (r != GABE_OK) {
+        std::string keep = g_err;
+        destroy_impl(c);
+        delete c;
+        g_err = keep;
+        return r;
+    }
+    *out = c;
+    return GABE_OK;
+}
+
+int gabe_destroy(gabe_ctx *c)
+{
+    if (!c) return GABE_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_impl(c);
+    delete c;
+    return GABE_OK;
+}
+
+int gabe_upload_state(gabe_ctx *c, const double *field, const double *dfield)
+{
+    if (!c || !field || !dfield) return fail(GABE_ERR_ARG, "NULL argument");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    CHK(upload_fields(c, c->fbuf[c->cur], field, d.pbar));
+    CHK(upload_fields(c, c->vbuf[c->cur], dfield, d.vbar));
+    d.e0_valid = 0;
+    CHK(write_scalars(c, d));
+    return GABE_OK;
+}
+
+int gabe_download_state(gabe_ctx *c, double *field, double *dfield)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    if (field) CHK(download_fields(c, c->fbuf[c->cur], field, d.pbar));
+    if (dfield) CHK(download_fields(c, c->vbuf[c->cur], dfield, d.vbar));
+    return GABE_OK;
+}
+
+int gabe_upload_gw(gabe_ctx *c, const double *h, const double *l)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    if (!c->alloc_gw) return fail(GABE_ERR_STATE, "context created with calc_gws = 0");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long M = c->M;
+    const double *src[2] = {h, l};
+    void *dst[2] = {c->h, c->l};
+    for (int a = 0; a < 2; a++) {
+        if (!src[a]) continue;
+        if (c->fp32) {
+            std::vector<float> tmp(12 * M);
+            for (int cc = 0; cc < 6; cc++)
+                for (long m = 0; m < M; m++) {
+                    tmp[2 * (cc * M + m)] = (float)src[a][(2 * cc) * M + m];
+                    tmp[2 * (cc * M + m) + 1] = (float)src[a][(2 * cc + 1) * M + m];
+                }
+            HIPCHK(hipMemcpy(dst[a], tmp.data(), sizeof(float) * 12 * M, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> tmp(12 * M);
+            for (int cc = 0; cc < 6; cc++)
+                for (long m = 0; m < M; m++) {
+                    tmp[2 * (cc * M + m)] = src[a][(2 * cc) * M + m];
+                    tmp[2 * (cc * M + m) + 1] = src[a][(2 * cc + 1) * M + m];
+                }
+            HIPCHK(hipMemcpy(dst[a], tmp.data(), sizeof(double) * 12 * M, hipMemcpyHostToDevice));
+        }
+    }
+    return GABE_OK;
+}
+
+int gabe_download_gw(gabe_ctx *c, double *h, double *l)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    if (!c->alloc_gw) return fail(GABE_ERR_STATE, "context created with calc_gws = 0");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long M = c->M;
+    double *dst[2] = {h, l};
+    const void *src[2] = {c->h, c->l};
+    for (int a = 0; a < 2; a++) {
+        if (!dst[a]) continue;
+        if (c->fp32) {
+            std::vector<float> tmp(12 * M);
+            HIPCHK(hipMemcpy(tmp.data(), src[a], sizeof(float) * 12 * M, hipMemcpyDeviceToHost));
+            for (int cc = 0; cc < 6; cc++)
+                for (long m = 0; m < M; m++) {
+                    dst[a][(2 * cc) * M + m] = tmp[2 * (cc * M + m)];
+                    dst[a][(2 * cc + 1) * M + m] = tmp[2 * (cc * M + m) + 1];
+                }
+        } else {
+            std::vector<double> tmp(12 * M);
+            HIPCHK(hipMemcpy(tmp.data(), src[a], sizeof(double) * 12 * M, hipMemcpyDeviceToHost));
+            for (int cc = 0; cc < 6; cc++)
+                for (long m = 0; m < M; m++) {
+                    dst[a][(2 * cc) * M + m] = tmp[2 * (cc * M + m)];
+                    dst[a][(2 * cc + 1) * M + m] = tmp[2 * (cc * M + m) + 1];
+                }
+        }
+    }
+    return GABE_OK;
+}
+
+int gabe_set_scalars(gabe_ctx *c, const gabe_scalars *s)
+{
+    if (!c || !s) return fail(GABE_ERR_ARG, "NULL argument");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    for (int k = 0; k < 2; k++) {
+        d.a[k] = s->a[k];
+        d.adot[k] = s->adot[k];
+        d.edkin[k] = s->edkin[k];
+        d.edpot[k] = s->edpot[k];
+        d.edgrad[k] = s->edgrad[k];
+        d.edrho[k] = s->edrho[k];
+    }
+    d.t = s->t;
+    d.nstep = s->nstep;
+    d.adot0_valid = 1;
+    d.e0_valid = 1;
+    d.error = 0;
+    CHK(write_scalars(c, d));
+    return GABE_OK;
+}
+
+int gabe_get_scalars(gabe_ctx *c, gabe_scalars *s)
+{
+    if (!c || !s) return fail(GABE_ERR_ARG, "NULL argument");
+    CHK(refresh_e0(c));
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    for (int k = 0; k < 2; k++) {
+        s->a[k] = d.a[k];
+        s->adot[k] = d.adot[k];
+        s->edkin[k] = d.edkin[k];
+        s->edpot[k] = d.edpot[k];
+        s->edgrad[k] = d.edgrad[k];
+        s->edrho[k] = d.edrho[k];
+    }
+    s->t = d.t;
+    s->nstep = d.nstep;
+    s->adot0_valid = d.adot0_valid;
+    s->error = d.error;
+    if (d.error) return fail(GABE_ERR_NONFINITE, "energy density became non-finite (unstable solution)");
+    return GABE_OK;
+}
+
+int gabe_energy(gabe_ctx *c, int s)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    if (s != 0) return fail(GABE_ERR_ARG, "only slot 0 holds a state between steps");
+    CHK(launch_sweep_ctx(c, SW_ENERGY, false));
+    CHK(launch_finalize_ctx(c, SW_ENERGY, 0));
+    return GABE_OK;
+}
+
+int gabe_step(gabe_ctx *c, int nsteps)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    if (nsteps < 0) return fail(GABE_ERR_ARG, "nsteps < 0");
+    for (int n = 0; n < nsteps; n++) CHK(one_step(c, nullptr));
+    return GABE_OK;
+}
+
+int gabe_sync(gabe_ctx *c)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    if (d.error) return fail(GABE_ERR_NONFINITE, "energy density became non-finite (unstable solution)");
+    return GABE_OK;
+}
+
+int gabe_spectra(gabe_ctx *c, double *out)
+{
+    if (!c || !out) return fail(GABE_ERR_ARG, "NULL argument");
+    CHK(refresh_e0(c));
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    const int nf = c->nf, N = c->N, nb = gabe_numbins(N);
+    HIPCHK(hipMemcpyAsync(c->stress, c->fbuf[c->cur], c->esz * nf * c->N3, hipMemcpyDeviceToDevice, c->stream));
+    CHK(run_plan(c, c->fft_spec, c->stress, c->shat));
+    if (c->fp32) {
+        SpecArgs<float> a{};
+        a.fhat = (const float *)c->shat;
+        a.l = (const float *)c->l;
+        for (int f = 0; f < nf; f++) a.dc[f] = d.pbar[f] * (double)c->N3;
+        a.nf = nf; a.N = N; a.numbins = nb; a.quirk = c->p.gw_spectrum_quirk; a.M = c->M; a.out = c->spec_out;
+        launch_spectra<float>(a, c->stream);
+    } else {
+        SpecArgs<double> a{};
+        a.fhat = (const double *)c->shat;
+        a.l = (const double *)c->l;
+        a.nf = nf; a.N = N; a.numbins = nb; a.quirk = c->p.gw_spectrum_quirk; a.M = c->M; a.out = c->spec_out;
+        launch_spectra<double>(a, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> raw((size_t)nb * (nf + 2));
+    HIPCHK(hipMemcpyAsync(raw.data(), c->spec_out, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+
+    /* normalisation of g2spectra.cpp:27,94-103,114-117,186-195 */
+    const double L = c->p.L;
+    const double spec_norm = std::pow(L / (double)N, 6.) * std::pow(2. * M_PI / L, 3.);
+    for (int f = 0; f < nf; f++) {
+        for (int b = 0; b < nb; b++) {
+            const double sum = raw[(size_t)b * (nf + 2) + f], cnt = raw[(size_t)b * (nf + 2) + nf];
+            if (cnt > 0) {
+                out[(2 * f) * nb + b] = sum * spec_norm / cnt;
+                out[(2 * f + 1) * nb + b] = std::pow((double)b * 2. * M_PI / L, 3) * sum * spec_norm / cnt / 2. / M_PI / M_PI;
+            } else {
+                out[(2 * f) * nb + b] = 0.;
+                out[(2 * f + 1) * nb + b] = 0.;
+            }
+        }
+    }
+    if (c->alloc_gw) {
+        const double hubtemp = std::sqrt(8. * M_PI / 3.) * std::sqrt(d.edrho[0]);
+        const double norm1_gw = 4.e-5 / std::pow(100., .333) / 24. / M_PI / M_PI / std::pow(L, 3.) / (hubtemp * hubtemp);
+        const int ncol = 2 * nf + 2;
+        for (int b = 0; b < nb; b++) {
+            const double sum = raw[(size_t)b * (nf + 2) + nf + 1], cnt = raw[(size_t)b * (nf + 2) + nf];
+            if (cnt > 0) {
+                out[(2 * nf) * nb + b] = sum * spec_norm / cnt;
+                out[(2 * nf + 1) * nb + b] = norm1_gw * std::pow((double)b * 2 * M_PI / L, 3.) * sum / cnt;
+            } else {
+                out[(2 * nf) * nb + b] = 0.;
+                out[(2 * nf + 1) * nb + b] = 0.;
+            }
+            out[ncol * nb + b] = 2. * M_PI / L * (double)b;
+        }
+    }
+    return GABE_OK;
+}
+
+int gabe_add_fluctuation(gabe_ctx *c, int fld, const double *fk, const double *fkd)
+{
+    if (!c || !fk || !fkd) return fail(GABE_ERR_ARG, "NULL argument");
+    if (fld < 0 || fld >= c->nf) return fail(GABE_ERR_ARG, "field index %d out of range", fld);
+    const int N = c->N, H = N / 2 + 1;
+    const long M = c->M, N3 = c->N3;
+    if (!c->fft_c2r.plan) CHK(make_plan(c, c->fft_c2r, rocfft_transform_type_real_inverse, true, 1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double *din = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc(&din, sizeof(double) * 2 * M));
+    HIPCHK(hipMalloc(&dout, sizeof(double) * N3));
+    std::vector<double> sym(2 * M);
+    const double *srcs[2] = {fk, fkd};
+    void *dsts[2] = {c->fbuf[c->cur], c->vbuf[c->cur]};
+    int rc = GABE_OK;
+    for (int a = 0; a < 2 && rc == GABE_OK; a++) {
+        const double *X = srcs[a];
+        std::memcpy(sym.data(), X, sizeof(double) * 2 * M);
+        /* Hermitian part of the kz = 0 and kz = N/2 planes: X <- (X(k) + conj X(-k)) / 2 */
+        for (int kz : {0, N / 2})
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < N; j++) {
+                    const long m = ((long)i * N + j) * H + kz;
+                    const long mc = ((long)((N - i) % N) * N + (N - j) % N) * H + kz;
+                    sym[2 * m] = 0.5 * (X[2 * m] + X[2 * mc]);
+                    sym[2 * m + 1] = 0.5 * (X[2 * m + 1] - X[2 * mc + 1]);
+                }
+        if (hipMemcpy(din, sym.data(), sizeof(double) * 2 * M, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(GABE_ERR_HIP, "upload of modes failed");
+            break;
+        }
+        rc = run_plan(c, c->fft_c2r, din, dout);
+        if (rc != GABE_OK) break;
+        if (c->fp32) launch_add_real<float>((float *)dsts[a] + (long)fld * N3, dout, N3, c->stream);
+        else launch_add_real<double>((double *)dsts[a] + (long)fld * N3, dout, N3, c->stream);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(GABE_ERR_HIP, "fluctuation add failed");
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (rc != GABE_OK) return rc;
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    d.e0_valid = 0;
+    CHK(write_scalars(c, d));
+    return GABE_OK;
+}
+
+int gabe_effmass(gabe_ctx *c, int fld, double *out)
+{
+    if (!c || !out) return fail(GABE_ERR_ARG, "NULL argument");
+    if (fld < 0 || fld >= c->nf) return fail(GABE_ERR_ARG, "field index %d out of range", fld);
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    const int nblk = 1024;
+    const long N3 = c->N3;
+    if (c->fp32) {
+        const float *f0 = (const float *)c->fbuf[c->cur];
+        launch_effmass<float>(f0, c->nf > 1 ? f0 + N3 : f0, c->nf, fld, d.pbar[0], c->nf > 1 ? d.pbar[1] : 0., c->model,
+                              N3, c->partials, nblk, c->stream);
+    } else {
+        const double *f0 = (const double *)c->fbuf[c->cur];
+        launch_effmass<double>(f0, c->nf > 1 ? f0 + N3 : f0, c->nf, fld, 0., 0., c->model, N3, c->partials, nblk,
+                               c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(nblk);
+    HIPCHK(hipMemcpyAsync(part.data(), c->partials, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = csum(part.data(), nblk) / (double)N3;
+    return GABE_OK;
+}
+
+int gabe_tt_source(gabe_ctx *c, int s, double *out)
+{
+    if (!c || !out) return fail(GABE_ERR_ARG, "NULL argument");
+    if (!c->alloc_gw) return fail(GABE_ERR_STATE, "context created with calc_gws = 0");
+    if (s != 0 && s != 1) return fail(GABE_ERR_ARG, "slot must be 0 or 1");
+    double *dout = nullptr;
+    HIPCHK(hipMalloc(&dout, sizeof(double) * 12 * c->M));
+    int rc = launch_sweep_ctx(c, SW_A, true);
+    if (rc == GABE_OK) rc = run_plan(c, c->fft_stress, c->stress, c->shat);
+    if (rc == GABE_OK) {
+        const double dx3 = c->dx * c->dx * c->dx;
+        if (c->fp32) launch_tt_out<float>((const float *)c->shat, s, c->N, c->M, dx3, dout, c->stream);
+        else launch_tt_out<double>((const double *)c->shat, s, c->N, c->M, dx3, dout, c->stream);
+        if (hipGetLastError() != hipSuccess) rc = fail(GABE_ERR_HIP, "tt kernel launch failed");
+    }
+    if (rc == GABE_OK && hipMemcpyAsync(out, dout, sizeof(double) * 12 * c->M, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(GABE_ERR_HIP, "download of T_gw failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == GABE_OK) rc = fail(GABE_ERR_HIP, "sync failed");
+    (void)hipFree(dout);
+    return rc;
+}
+
+int gabe_history(gabe_ctx *c, double *out, int max_rows)
+{
+    if (!c || !out || max_rows < 0) return fail(GABE_ERR_ARG, "bad argument");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    std::vector<double> h((size_t)c->hist_cap * GABE_HIST_COLS);
+    HIPCHK(hipMemcpy(h.data(), c->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    const long total = (long)d.nstep;
+    const long avail = std::min<long>(total, c->hist_cap);
+    const long nrow = std::min<long>(avail, max_rows);
+    for (long r = 0; r < nrow; r++) {
+        const long stepno = total - nrow + r;
+        std::memcpy(out + r * GABE_HIST_COLS, h.data() + (stepno % c->hist_cap) * GABE_HIST_COLS,
+                    sizeof(double) * GABE_HIST_COLS);
+    }
+    return (int)nrow;
+}
+
+int gabe_kernel_times(gabe_ctx *c, int nsteps, double *ms_out)
+{
+    if (!c || !ms_out || nsteps <= 0) return fail(GABE_ERR_ARG, "bad argument");
+    const int nev = 2 * GABE_NKCLASS;
+    std::vector<hipEvent_t> ev((size_t)nev * nsteps);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = GABE_OK;
+    for (int n = 0; n < nsteps && rc == GABE_OK; n++) rc = one_step(c, ev.data() + (size_t)n * nev);
+    if (rc == GABE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GABE_ERR_HIP, "sync failed");
+    for (int k = 0; k < GABE_NKCLASS; k++) ms_out[k] = 0.;
+    if (rc == GABE_OK) {
+        for (int n = 0; n < nsteps; n++)
+            for (int k = 0; k < GABE_NKCLASS; k++) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, ev[(size_t)n * nev + 2 * k], ev[(size_t)n * nev + 2 * k + 1]);
+                ms_out[k] += ms / nsteps;
+            }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+} // extern "C"

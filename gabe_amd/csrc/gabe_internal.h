@@ -1,0 +1,110 @@
+/*
+ * gabe_internal.h -- device-side data structures and kernel launchers shared
+ * by the kernels (gabe_kernels.hip) and the C-ABI implementation (gabe_api.hip).
+ */
+#ifndef GABE_INTERNAL_H
+#define GABE_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include "gabe_model.h"
+
+#define GABE_MAXF 2
+#define GABE_BLOCK 256
+#define GABE_HIST_COLS 8
+
+/* Persistent device scalars: the reference's a[2], adot[2], ed*[2]
+ * (g2header.h:59-64) plus, per field, the FP64 background (mean field pbar and
+ * mean velocity vbar) around which the FP32 form stores its fluctuations. */
+struct DevScalars {
+    double a[2], adot[2], edkin[2], edpot[2], edgrad[2], edrho[2];
+    double pbar[GABE_MAXF], vbar[GABE_MAXF];
+    double t;
+    long long nstep;
+    int adot0_valid; /* adot[0] belongs to the current state */
+    int e0_valid;    /* ed*[0] belong to the current state */
+    int error;       /* non-finite energy density seen */
+    int pad;
+};
+
+/* Values frozen by the step's finalize for sweep B and the GW mode kernel. */
+struct StepScalars {
+    double a0, adot0, a1, adot1;
+    double pbar0[GABE_MAXF], vbar0[GABE_MAXF], pbar1[GABE_MAXF], vbar1[GABE_MAXF];
+    double meanVp0[GABE_MAXF], meanVp1[GABE_MAXF];
+};
+
+/* Partial sums emitted per sweep workgroup.  Index layout:
+ *   0: sum V(f0)    1: sum V(f1)
+ *   per field f, base 2 + 7 f:
+ *     +0 sum fdot0^2   +1 sum fdot0 X   +2 sum X^2   (X = lap f0/a0^2 - V'(f0))
+ *     +3 sum f0 lap f0 +4 sum f1 lap f1
+ *     +5 sum V'(f0)    +6 sum V'(f1)
+ * f0 is the step-start state, f1 = f0 + dt/2 fdot0 the midpoint field. */
+__host__ __device__ constexpr int gabe_nsums(int nf) { return 2 + 7 * nf; }
+#define GABE_MAXSUMS (2 + 7 * GABE_MAXF)
+
+enum SweepMode { SW_ENERGY = 0, SW_A = 1, SW_B = 2 };
+
+template <typename T>
+struct SweepArgs {
+    const T *f[GABE_MAXF];  /* field (FP64) or fluctuation (FP32), input state */
+    const T *v[GABE_MAXF];  /* time derivative, input state */
+    T *fo[GABE_MAXF];       /* sweep B output state */
+    T *vo[GABE_MAXF];
+    T *stress;              /* [12][N^3]: S(f0) comps 0-5, S(f1) comps 6-11 */
+    double *partials;       /* [nsums][nblk] */
+    const DevScalars *sc;
+    const StepScalars *ss;
+    gabe_model model;
+    int N, ci, nblk;
+    long N2, N3;
+    double dt, inv_dx2;
+};
+
+struct FinArgs {
+    const double *partials;
+    int nblk, nf, mode, exp_type, split, refresh_adot;
+    double N3, dt;
+    DevScalars *sc;
+    StepScalars *ss;
+    double *hist;
+    int hist_cap;
+};
+
+template <typename T>
+struct GwArgs {
+    const T *shat;      /* [12][M] complex: S(f0) transforms 0-5, S(f1) 6-11 */
+    T *h, *l;           /* [6][M] complex, components 11,12,13,22,23,33 */
+    const StepScalars *ss;
+    int N;
+    long M;
+    double L, dt, dx3;
+};
+
+template <typename T>
+struct SpecArgs {
+    const T *fhat;      /* [nf][M] complex field transforms */
+    const T *l;         /* [6][M] complex, or null */
+    double dc[GABE_MAXF];   /* added to the real part of each field's k=0 mode */
+    int nf, N, numbins, quirk;
+    long M;
+    double *out;        /* [numbins][nf + 2]: per-field sum w|F|^2, sum w, sum w E_gw */
+};
+
+template <typename T>
+void launch_sweep(int mode, int nf, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s);
+void launch_finalize(const FinArgs &a, hipStream_t s);
+template <typename T>
+void launch_gw(const GwArgs<T> &a, hipStream_t s);
+template <typename T>
+void launch_tt_out(const T *shat, int stage, int N, long M, double dx3, double *out, hipStream_t s);
+template <typename T>
+void launch_spectra(const SpecArgs<T> &a, hipStream_t s);
+template <typename T>
+void launch_effmass(const T *f0, const T *f1, int nf, int fld, double pbar0, double pbar1,
+                    gabe_model m, long n3, double *partials, int nblk, hipStream_t s);
+template <typename T>
+void launch_add_real(T *dst, const double *add, long n, hipStream_t s);
+
+#endif

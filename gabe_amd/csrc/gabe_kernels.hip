@@ -1,0 +1,651 @@
+/*
+ * gabe_kernels.hip -- CDNA4 (gfx950) kernels of the lattice evolution.
+ *
+ * One step (expansion_type 1, step(), g2functions.cpp:234-270) is
+ *   sweep A  -> finalize -> [R2C of the 12 stress arrays -> GW mode kernel]
+ *                        -> sweep B
+ * Sweep A reads the step-start state once (7-point halo) and produces every
+ * quantity that needs a global reduction: the energies of the step-start state
+ * (the reference's end-of-step calcEnergy(0), g2functions.cpp:269-270), the
+ * midpoint energies (calcEnergy(1), :251) and the mean potential forces, plus
+ * the gradient stress of both RK2 stages for the tensor source
+ * (g2evolutionpert.cpp:165-180).  The FP64 finalize turns the partial sums into
+ * a, adot and the energies on the device.  Sweep B recomputes the midpoint from
+ * the same 1-deep halo and writes the new state.  The GW mode kernel fuses the
+ * TT projection and both RK2 stages of every tensor mode.
+ *
+ * Storage type T is float (FP32 fluctuations around an FP64 background) or
+ * double (full fields, the FP64 form).  Potentials, forces and every
+ * reduction are FP64 in both; the stencils and stress accumulate in T.
+ */
+#include <hip/hip_runtime.h>
+
+#include "gabe_internal.h"
+
+namespace {
+
+template <typename T>
+struct cplx_of;
+template <>
+struct cplx_of<float> { using type = float2; };
+template <>
+struct cplx_of<double> { using type = double2; };
+
+/* midpoint field value f + (dt/2) fdot, one rounding; sweeps A and B use this
+ * same function so both see identical midpoint fields */
+template <typename T>
+__device__ __forceinline__ T midpt(T f, T v, T hdt) { return fma(v, hdt, f); }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+/* fixed-order block reduction of NS sums; one partial per workgroup per sum */
+template <int NS>
+__device__ void block_store_partials(double (&acc)[NS], double *partials, int nblk, int bid)
+{
+    __shared__ double red[GABE_BLOCK / 64][NS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double v = wave_sum(acc[s]);
+        if (lane == 0) red[w][s] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        const int s = threadIdx.x;
+        partials[(long)s * nblk + bid] = (red[0][s] + red[1][s]) + (red[2][s] + red[3][s]);
+    }
+}
+
+/* ------------------------------------------------------------------ sweeps */
+/* Thread layout: one thread per (j,k) column of the N x N plane, marching
+ * through ci consecutive i-planes; the i-1/i/i+1 values stay in registers,
+ * the four lateral neighbours come through the L1/L2 caches (they are the
+ * centre loads of the adjacent lanes and rows).  Consecutive lanes read
+ * consecutive k: every load is coalesced. */
+template <typename T, int NF, int MODE, bool GW>
+__global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
+{
+    constexpr int NS = gabe_nsums(NF);
+    const int N = A.N;
+    const long N2 = A.N2;
+    const long p = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    const bool valid = p < N2;
+    const long pp = valid ? p : 0;
+    const int j = (int)(pp / N), k = (int)(pp - (long)j * N);
+    const long pkp = (long)j * N + (k + 1 == N ? 0 : k + 1);
+    const long pkm = (long)j * N + (k == 0 ? N - 1 : k - 1);
+    const long pjp = (long)(j + 1 == N ? 0 : j + 1) * N + k;
+    const long pjm = (long)(j == 0 ? N - 1 : j - 1) * N + k;
+    const int i0 = blockIdx.y * A.ci;
+    const int i1 = min(i0 + A.ci, N);
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+
+    const T idx2 = (T)A.inv_dx2;
+    const double dt = A.dt, hdt = 0.5 * dt;
+    const T hdtT = (T)hdt;
+
+    /* scalars: sweep A / energy read the current state, sweep B the frozen step values */
+    double pb0[2] = {0., 0.}, pb1[2] = {0., 0.}, vb0[2] = {0., 0.}, mvp0[2] = {0., 0.}, mvp1[2] = {0., 0.};
+    double ia02, ia12 = 0., H0 = 0., H1 = 0.;
+    if (MODE == SW_B) {
+        const StepScalars *ss = A.ss;
+        ia02 = 1. / (ss->a0 * ss->a0);
+        ia12 = 1. / (ss->a1 * ss->a1);
+        H0 = ss->adot0 / ss->a0;
+        H1 = ss->adot1 / ss->a1;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            pb0[f] = ss->pbar0[f]; pb1[f] = ss->pbar1[f];
+            mvp0[f] = ss->meanVp0[f]; mvp1[f] = ss->meanVp1[f];
+        }
+    } else {
+        const DevScalars *sc = A.sc;
+        ia02 = 1. / (sc->a[0] * sc->a[0]);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            pb0[f] = sc->pbar[f];
+            vb0[f] = sc->vbar[f];
+            pb1[f] = sc->pbar[f] + hdt * sc->vbar[f];
+        }
+    }
+
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) acc[s] = 0.;
+
+    T fm[NF], fc[NF], vm[NF], vc[NF];
+    const int im0 = i0 == 0 ? N - 1 : i0 - 1;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        fm[f] = A.f[f][(long)im0 * N2 + pp];
+        fc[f] = A.f[f][(long)i0 * N2 + pp];
+        vm[f] = A.v[f][(long)im0 * N2 + pp];
+        vc[f] = A.v[f][(long)i0 * N2 + pp];
+    }
+
+    for (int i = i0; i < i1; i++) {
+        const int ip = i + 1 == N ? 0 : i + 1;
+        const long base = (long)i * N2;
+        T fp[NF], vp[NF];
+        T lap0[NF], lap1[NF], c1[NF];
+        T g0[NF][3], g1[NF][3];
+        double full0[2] = {0., 0.}, full1[2] = {0., 0.};
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            fp[f] = A.f[f][(long)ip * N2 + pp];
+            vp[f] = A.v[f][(long)ip * N2 + pp];
+            const T fkp = A.f[f][base + pkp], fkm = A.f[f][base + pkm];
+            const T fjp = A.f[f][base + pjp], fjm = A.f[f][base + pjm];
+            lap0[f] = (((fm[f] + fp[f]) + (fjm + fjp)) + (fkm + fkp) - (T)6 * fc[f]) * idx2;
+            full0[f] = pb0[f] + (double)fc[f];
+            if (MODE != SW_ENERGY) {
+                const T vkp = A.v[f][base + pkp], vkm = A.v[f][base + pkm];
+                const T vjp = A.v[f][base + pjp], vjm = A.v[f][base + pjm];
+                const T m1 = midpt(fm[f], vm[f], hdtT), p1 = midpt(fp[f], vp[f], hdtT);
+                const T jm1 = midpt(fjm, vjm, hdtT), jp1 = midpt(fjp, vjp, hdtT);
+                const T km1 = midpt(fkm, vkm, hdtT), kp1 = midpt(fkp, vkp, hdtT);
+                c1[f] = midpt(fc[f], vc[f], hdtT);
+                lap1[f] = (((m1 + p1) + (jm1 + jp1)) + (km1 + kp1) - (T)6 * c1[f]) * idx2;
+                full1[f] = pb1[f] + (double)c1[f];
+                if (GW) {
+                    g0[f][0] = (fp[f] - fm[f]) * (T)0.5;
+                    g0[f][1] = (fjp - fjm) * (T)0.5;
+                    g0[f][2] = (fkp - fkm) * (T)0.5;
+                    g1[f][0] = (p1 - m1) * (T)0.5;
+                    g1[f][1] = (jp1 - jm1) * (T)0.5;
+                    g1[f][2] = (kp1 - km1) * (T)0.5;
+                }
+            }
+        }
+
+        if (valid) {
+            if (MODE == SW_B) {
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    const double Vp0 = gabe_dVdf(A.model, f, full0[0], full0[1]);
+                    const double Vp1 = gabe_dVdf(A.model, f, full1[0], full1[1]);
+                    const double dv0 = (double)vc[f];
+                    /* F(0) and F(1) of ddfield (g2functions.cpp:193-196) minus the
+                     * background force (zero in the FP64 form) */
+                    const double dF0 = (double)lap0[f] * ia02 - (Vp0 - mvp0[f]) - 3. * H0 * dv0;
+                    const double dv1 = dv0 + hdt * dF0;
+                    const double dF1 = (double)lap1[f] * ia12 - (Vp1 - mvp1[f]) - 3. * H1 * dv1;
+                    A.fo[f][base + pp] = (T)((double)fc[f] + dt * dv1);
+                    A.vo[f][base + pp] = (T)(dv0 + dt * dF1);
+                }
+            } else {
+                acc[0] += gabe_V(A.model, full0[0], full0[1]);
+                if (MODE == SW_A) acc[1] += gabe_V(A.model, full1[0], full1[1]);
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    const double vel0 = vb0[f] + (double)vc[f];
+                    const int b = 2 + 7 * f;
+                    acc[b + 0] += vel0 * vel0;
+                    acc[b + 3] += (double)fc[f] * (double)lap0[f];
+                    if (MODE == SW_A) {
+                        const double Vp0 = gabe_dVdf(A.model, f, full0[0], full0[1]);
+                        const double Vp1 = gabe_dVdf(A.model, f, full1[0], full1[1]);
+                        const double X = (double)lap0[f] * ia02 - Vp0;
+                        acc[b + 1] += vel0 * X;
+                        acc[b + 2] += X * X;
+                        acc[b + 4] += (double)c1[f] * (double)lap1[f];
+                        acc[b + 5] += Vp0;
+                        acc[b + 6] += Vp1;
+                    }
+                }
+                if (MODE == SW_A && GW) {
+                    /* traceless gradient stress (g2evolutionpert.cpp:48-132):
+                     * S_ab = sum_f D_a f D_b f / dx^2 - delta_ab/3 sum_f |D f|^2 / dx^2 */
+#pragma unroll
+                    for (int st = 0; st < 2; st++) {
+                        T sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+#pragma unroll
+                        for (int f = 0; f < NF; f++) {
+                            const T *g = st == 0 ? g0[f] : g1[f];
+                            sxx += g[0] * g[0]; syy += g[1] * g[1]; szz += g[2] * g[2];
+                            sxy += g[0] * g[1]; sxz += g[0] * g[2]; syz += g[1] * g[2];
+                        }
+                        const T q3 = (sxx + syy + szz) * (T)(1. / 3.);
+                        T *S = A.stress + (long)(6 * st) * A.N3 + base + pp;
+                        S[0 * A.N3] = (sxx - q3) * idx2;
+                        S[1 * A.N3] = sxy * idx2;
+                        S[2 * A.N3] = sxz * idx2;
+                        S[3 * A.N3] = (syy - q3) * idx2;
+                        S[4 * A.N3] = syz * idx2;
+                        S[5 * A.N3] = (szz - q3) * idx2;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            fm[f] = fc[f]; fc[f] = fp[f];
+            vm[f] = vc[f]; vc[f] = vp[f];
+        }
+    }
+    if (MODE != SW_B) block_store_partials<NS>(acc, A.partials, A.nblk, bid);
+}
+
+/* ---------------------------------------------------------------- finalize */
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+/* One workgroup: compensated fixed-order sum of the partials, then the
+ * expansion update of g2functions.cpp:250-252,268-270 and the background
+ * update of the FP32 form.  Deterministic: the same partials give the same
+ * bits on every run. */
+__global__ __launch_bounds__(GABE_BLOCK) void k_finalize(FinArgs F)
+{
+    const int ns = gabe_nsums(F.nf);
+    double s[GABE_MAXSUMS], c[GABE_MAXSUMS];
+    for (int q = 0; q < GABE_MAXSUMS; q++) { s[q] = 0.; c[q] = 0.; }
+    for (int b = threadIdx.x; b < F.nblk; b += GABE_BLOCK) {
+        for (int q = 0; q < ns; q++) {
+            double t, e;
+            two_sum(s[q], F.partials[(long)q * F.nblk + b], t, e);
+            s[q] = t;
+            c[q] += e;
+        }
+    }
+    __shared__ double sh_s[GABE_BLOCK], sh_c[GABE_BLOCK];
+    __shared__ double tot[GABE_MAXSUMS];
+    for (int q = 0; q < ns; q++) {
+        sh_s[threadIdx.x] = s[q];
+        sh_c[threadIdx.x] = c[q];
+        __syncthreads();
+        for (int w = GABE_BLOCK / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                double t, e;
+                two_sum(sh_s[threadIdx.x], sh_s[threadIdx.x + w], t, e);
+                sh_s[threadIdx.x] = t;
+                sh_c[threadIdx.x] = sh_c[threadIdx.x] + sh_c[threadIdx.x + w] + e;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[q] = sh_s[0] + sh_c[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+
+    DevScalars *sc = F.sc;
+    const double N3 = F.N3, dt = F.dt;
+    const double a0 = sc->a[0];
+    double K0 = 0., G0 = 0.;
+    for (int f = 0; f < F.nf; f++) {
+        K0 += tot[2 + 7 * f + 0];
+        G0 += tot[2 + 7 * f + 3];
+    }
+    const double edkin0 = K0 / N3 / 2.;
+    const double edpot0 = tot[0] / N3;
+    const double edgrad0 = -G0 / N3 / 2. / a0 / a0;
+    const double edrho0 = edkin0 + edpot0 + edgrad0;
+
+    if (F.mode == SW_ENERGY) {
+        sc->edkin[0] = edkin0; sc->edpot[0] = edpot0; sc->edgrad[0] = edgrad0; sc->edrho[0] = edrho0;
+        sc->e0_valid = 1;
+        if (F.refresh_adot && !sc->adot0_valid) {
+            sc->adot[0] = F.exp_type == 0 ? 0. : sqrt(8. * M_PI / 3. * edrho0) * a0;
+            sc->adot0_valid = 1;
+        }
+        if (!isfinite(edrho0)) sc->error = 1;
+        return;
+    }
+
+    /* adot0 = adf(0) of the step-start state (:270 of the previous step), or
+     * the value set by the host for the first step (initexpansion, Q4) */
+    double adot0;
+    if (F.exp_type == 0) adot0 = 0.;
+    else if (sc->adot0_valid) adot0 = sc->adot[0];
+    else adot0 = sqrt(8. * M_PI / 3. * edrho0) * a0;
+    const double H0 = adot0 / a0;
+
+    /* midpoint kinetic energy from the expansion of
+     * fdot1 = fdot0 + dt/2 (X - 3 H0 fdot0) = cf fdot0 + dt/2 X */
+    const double cf = 1. - 1.5 * dt * H0;
+    double K1 = 0., G1 = 0.;
+    for (int f = 0; f < F.nf; f++) {
+        const int b = 2 + 7 * f;
+        K1 += cf * cf * tot[b] + cf * dt * tot[b + 1] + 0.25 * dt * dt * tot[b + 2];
+        G1 += tot[b + 4];
+    }
+    const double a1 = F.exp_type == 0 ? a0 : a0 + .5 * dt * adot0;           /* :250 */
+    const double edkin1 = K1 / N3 / 2.;
+    const double edpot1 = tot[1] / N3;
+    const double edgrad1 = -G1 / N3 / 2. / a1 / a1;
+    const double edrho1 = edkin1 + edpot1 + edgrad1;                          /* :251 */
+    const double adot1 = F.exp_type == 0 ? 0. : sqrt(8. * M_PI / 3. * edrho1) * a1; /* :252 */
+    const double H1 = adot1 / a1;
+    const double a0n = F.exp_type == 0 ? a0 : a0 + dt * adot1;                /* :268 */
+
+    StepScalars *ss = F.ss;
+    ss->a0 = a0; ss->adot0 = adot0; ss->a1 = a1; ss->adot1 = adot1;
+    for (int f = 0; f < F.nf; f++) {
+        const int b = 2 + 7 * f;
+        const double mv0 = F.split ? tot[b + 5] / N3 : 0.;
+        const double mv1 = F.split ? tot[b + 6] / N3 : 0.;
+        const double pb0 = sc->pbar[f], vb0 = sc->vbar[f];
+        const double pb1 = pb0 + .5 * dt * vb0;
+        const double vb1 = vb0 + .5 * dt * (-mv0 - 3. * H0 * vb0);
+        ss->pbar0[f] = pb0; ss->vbar0[f] = vb0; ss->pbar1[f] = pb1; ss->vbar1[f] = vb1;
+        ss->meanVp0[f] = mv0; ss->meanVp1[f] = mv1;
+        sc->pbar[f] = pb0 + dt * vb1;
+        sc->vbar[f] = vb0 + dt * (-mv1 - 3. * H1 * vb1);
+    }
+    if (F.hist && F.hist_cap > 0) {
+        double *row = F.hist + (sc->nstep % F.hist_cap) * GABE_HIST_COLS;
+        row[0] = sc->t; row[1] = a0; row[2] = adot0; row[3] = edkin0;
+        row[4] = edpot0; row[5] = edgrad0; row[6] = edrho0; row[7] = (double)sc->nstep;
+    }
+    sc->edkin[0] = edkin0; sc->edpot[0] = edpot0; sc->edgrad[0] = edgrad0; sc->edrho[0] = edrho0;
+    sc->edkin[1] = edkin1; sc->edpot[1] = edpot1; sc->edgrad[1] = edgrad1; sc->edrho[1] = edrho1;
+    sc->a[0] = a0n; sc->a[1] = a1;
+    sc->adot[0] = adot0; sc->adot[1] = adot1;
+    sc->t += dt;
+    sc->nstep += 1;
+    sc->adot0_valid = 0;
+    sc->e0_valid = 0;
+    if (!isfinite(edrho0) || !isfinite(edrho1)) sc->error = 1;
+}
+
+/* ------------------------------------------------------------- GW modes */
+template <typename C>
+__device__ __forceinline__ C cmul(C a, double s) { a.x *= s; a.y *= s; return a; }
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
+
+/* T = Lambda(n) S dx^3 with Lambda_ij,lm = P_il P_jm - P_ij P_lm / 2 and
+ * P = 1 - n n^T, evaluated as T = S - n u^T - u n^T + w n n^T - P (tr S - w)/2
+ * with u = S n, w = n.S.n.  Same operator as g2evolutionpert.cpp:215-310.
+ * Components ordered 11,12,13,22,23,33. */
+template <typename C, typename R>
+__device__ __forceinline__ void tt_project(const C S[6], R nx, R ny, R nz, R dx3, C T[6])
+{
+    C u[3];
+    u[0].x = S[0].x * nx + S[1].x * ny + S[2].x * nz;
+    u[0].y = S[0].y * nx + S[1].y * ny + S[2].y * nz;
+    u[1].x = S[1].x * nx + S[3].x * ny + S[4].x * nz;
+    u[1].y = S[1].y * nx + S[3].y * ny + S[4].y * nz;
+    u[2].x = S[2].x * nx + S[4].x * ny + S[5].x * nz;
+    u[2].y = S[2].y * nx + S[4].y * ny + S[5].y * nz;
+    C w;
+    w.x = u[0].x * nx + u[1].x * ny + u[2].x * nz;
+    w.y = u[0].y * nx + u[1].y * ny + u[2].y * nz;
+    C hq;   /* (tr S - w) / 2 */
+    hq.x = (S[0].x + S[3].x + S[5].x - w.x) * (R)0.5;
+    hq.y = (S[0].y + S[3].y + S[5].y - w.y) * (R)0.5;
+    const R n[3] = {nx, ny, nz};
+    const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const R ni = n[ia[c]], nj = n[ib[c]];
+        const R pij = (ia[c] == ib[c] ? (R)1 : (R)0) - ni * nj;
+        T[c].x = (S[c].x - ni * u[ib[c]].x - u[ia[c]].x * nj + w.x * ni * nj - pij * hq.x) * dx3;
+        T[c].y = (S[c].y - ni * u[ib[c]].y - u[ia[c]].y * nj + w.y * ni * nj - pij * hq.y) * dx3;
+    }
+}
+
+/* One thread per mode: TT source of both stages, then the two midpoint-RK2
+ * stages of evolve_perts (g2evolutionpert.cpp:343-455, derivs :459-516) in
+ * registers; reads S0, S1, h, l once and writes h, l once. */
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_gw(GwArgs<T> G)
+{
+    using C = typename cplx_of<T>::type;
+    const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    if (m >= G.M) return;
+    const int N = G.N, H = N / 2 + 1;
+    const int kz = (int)(m % H);
+    const long r = m / H;
+    const int j = (int)(r % N), i = (int)(r / N);
+    const int px = i <= N / 2 ? i : i - N, py = j <= N / 2 ? j : j - N, pz = kz;
+    const int p2 = px * px + py * py + pz * pz;
+
+    const C *Sh = reinterpret_cast<const C *>(G.shat);
+    C *hh = reinterpret_cast<C *>(G.h);
+    C *ll = reinterpret_cast<C *>(G.l);
+    C S0[6], S1[6], h0[6], l0[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        S0[c] = Sh[(long)c * G.M + m];
+        S1[c] = Sh[(long)(6 + c) * G.M + m];
+        h0[c] = hh[(long)c * G.M + m];
+        l0[c] = ll[(long)c * G.M + m];
+    }
+    C T0[6], T1[6];
+    if (p2 == 0) {   /* zero mode (g2evolutionpert.cpp:319-321) */
+#pragma unroll
+        for (int c = 0; c < 6; c++) { T0[c].x = T0[c].y = 0; T1[c].x = T1[c].y = 0; }
+    } else {
+        const double inv = 1. / sqrt((double)p2);
+        const T nx = (T)(px * inv), ny = (T)(py * inv), nz = (T)(pz * inv);
+        tt_project<C, T>(S0, nx, ny, nz, (T)G.dx3, T0);
+        tt_project<C, T>(S1, nx, ny, nz, (T)G.dx3, T1);
+    }
+    const StepScalars *ss = G.ss;
+    const double k2n = 4. * M_PI * M_PI / G.L / G.L * (double)p2;
+    const T kk0 = (T)(k2n / (ss->a0 * ss->a0)), kk1 = (T)(k2n / (ss->a1 * ss->a1));
+    const T fr0 = (T)(3. * ss->adot0 / ss->a0), fr1 = (T)(3. * ss->adot1 / ss->a1);
+    const T om0 = (T)(16. * M_PI / (ss->a0 * ss->a0)), om1 = (T)(16. * M_PI / (ss->a1 * ss->a1));
+    const T hdt = (T)(0.5 * G.dt), dt = (T)G.dt;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        C h1, l1, hn, ln;
+        h1.x = h0[c].x + hdt * l0[c].x;
+        h1.y = h0[c].y + hdt * l0[c].y;
+        l1.x = l0[c].x + hdt * (-kk0 * h0[c].x - fr0 * l0[c].x + om0 * T0[c].x);
+        l1.y = l0[c].y + hdt * (-kk0 * h0[c].y - fr0 * l0[c].y + om0 * T0[c].y);
+        hn.x = h0[c].x + dt * l1.x;
+        hn.y = h0[c].y + dt * l1.y;
+        ln.x = l0[c].x + dt * (-kk1 * h1.x - fr1 * l1.x + om1 * T1[c].x);
+        ln.y = l0[c].y + dt * (-kk1 * h1.y - fr1 * l1.y + om1 * T1[c].y);
+        hh[(long)c * G.M + m] = hn;
+        ll[(long)c * G.M + m] = ln;
+    }
+}
+
+/* T_gw[s] in the reference layout [12][M] (diagnostic, not on the step path) */
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, int stage, int N, long M, double dx3, double *out)
+{
+    using C = typename cplx_of<T>::type;
+    const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    if (m >= M) return;
+    const int H = N / 2 + 1;
+    const int kz = (int)(m % H);
+    const long r = m / H;
+    const int j = (int)(r % N), i = (int)(r / N);
+    const int px = i <= N / 2 ? i : i - N, py = j <= N / 2 ? j : j - N, pz = kz;
+    const int p2 = px * px + py * py + pz * pz;
+    const C *Sh = reinterpret_cast<const C *>(shat) + (long)(6 * stage) * M;
+    C S[6], Tt[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) S[c] = Sh[(long)c * M + m];
+    if (p2 == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) Tt[c].x = Tt[c].y = 0;
+    } else {
+        const double inv = 1. / sqrt((double)p2);
+        tt_project<C, T>(S, (T)(px * inv), (T)(py * inv), (T)(pz * inv), (T)dx3, Tt);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        out[(long)(2 * c) * M + m] = (double)Tt[c].x;
+        out[(long)(2 * c + 1) * M + m] = (double)Tt[c].y;
+    }
+}
+
+/* ------------------------------------------------------------- spectra */
+__device__ __forceinline__ int isqrt_floor(long x)
+{
+    if (x <= 0) return 0;
+    long r = (long)sqrt((double)x);
+    while (r * r > x) r--;
+    while ((r + 1) * (r + 1) <= x) r++;
+    return (int)r;
+}
+
+/* One workgroup per bin b: the modes with floor(|n|) = b (g2spectra.cpp:68-92,
+ * 126-184) are enumerated column by column ((px,py) fixed, kz contiguous), so
+ * every bin is summed by one workgroup in a fixed order: the result does not
+ * depend on scheduling.  Weights: 2 for 1 <= kz <= N/2-1, 1 for kz in {0, N/2}. */
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_spectra(SpecArgs<T> P)
+{
+    using C = typename cplx_of<T>::type;
+    const int b = blockIdx.x;
+    const int N = P.N, H = N / 2 + 1, nf = P.nf;
+    const long M = P.M;
+    const C *F = reinterpret_cast<const C *>(P.fhat);
+    const C *Lg = reinterpret_cast<const C *>(P.l);
+    const int R = b + 1;
+    const int lo = max(-R, -N / 2 + 1), hi = min(R, N / 2);
+    const int W = hi - lo + 1;
+    const long b2 = (long)b * b, r2max = (long)(b + 1) * (b + 1) - 1;
+    double acc[GABE_MAXF + 2] = {0., 0., 0., 0.};
+    for (long q = threadIdx.x; q < (long)W * W; q += GABE_BLOCK) {
+        const int px = lo + (int)(q / W), py = lo + (int)(q % W);
+        const long r2 = (long)px * px + (long)py * py;
+        if (r2 > r2max) continue;
+        int kzlo = 0;
+        if (b2 - r2 > 0) {
+            kzlo = isqrt_floor(b2 - r2);
+            if ((long)kzlo * kzlo < b2 - r2) kzlo++;
+        }
+        const int kzhi = min(isqrt_floor(r2max - r2), N / 2);
+        const int i = px >= 0 ? px : px + N, j = py >= 0 ? py : py + N;
+        const long row = ((long)i * N + j) * H;
+        for (int kz = kzlo; kz <= kzhi; kz++) {
+            const double w = (kz == 0 || kz == N / 2) ? 1. : 2.;
+            const long m = row + kz;
+            for (int f = 0; f < nf; f++) {
+                const C v = F[(long)f * M + m];
+                double re = (double)v.x, im = (double)v.y;
+                if (m == 0) re += P.dc[f];
+                acc[f] += w * (re * re + im * im);
+            }
+            acc[nf] += w;
+            if (Lg) {
+                const long ms = (P.quirk && kz == 0) ? row + N / 2 : m;
+                double e = 0.;
+#pragma unroll
+                for (int c = 0; c < 6; c++) {
+                    const C v = Lg[(long)c * M + ms];
+                    const double cw = (c == 0 || c == 3 || c == 5) ? 1. : 2.;
+                    e += cw * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+                }
+                acc[nf + 1] += w * e;
+            }
+        }
+    }
+    __shared__ double red[GABE_BLOCK / 64][GABE_MAXF + 2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < nf + 2; q++) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) red[wv][q] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nf + 2) {
+        const int q = threadIdx.x;
+        P.out[(long)b * (nf + 2) + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    }
+}
+
+/* --------------------------------------------------------------- helpers */
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_effmass(const T *f0, const T *f1, int nf, int fld, double pbar0,
+                                                          double pbar1, gabe_model m, long n3, double *partials)
+{
+    double acc[1] = {0.};
+    for (long n = (long)blockIdx.x * GABE_BLOCK + threadIdx.x; n < n3; n += (long)gridDim.x * GABE_BLOCK) {
+        const double phi = pbar0 + (double)f0[n];
+        const double chi = nf > 1 ? pbar1 + (double)f1[n] : 0.;
+        acc[0] += gabe_d2Vdf2(m, fld, phi, chi);
+    }
+    block_store_partials<1>(acc, partials, gridDim.x, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_add_real(T *dst, const double *add, long n)
+{
+    const long i = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    if (i < n) dst[i] = (T)((double)dst[i] + add[i]);
+}
+
+} // namespace
+
+/* ---------------------------------------------------------------- launchers */
+template <typename T, int NF>
+static void launch_sweep_nf(int mode, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s)
+{
+    if (mode == SW_ENERGY) hipLaunchKernelGGL((k_sweep<T, NF, SW_ENERGY, false>), grid, dim3(GABE_BLOCK), 0, s, a);
+    else if (mode == SW_A && gw) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, true>), grid, dim3(GABE_BLOCK), 0, s, a);
+    else if (mode == SW_A) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, false>), grid, dim3(GABE_BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_sweep<T, NF, SW_B, false>), grid, dim3(GABE_BLOCK), 0, s, a);
+}
+
+template <typename T>
+void launch_sweep(int mode, int nf, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s)
+{
+    if (nf == 1) launch_sweep_nf<T, 1>(mode, gw, a, grid, s);
+    else launch_sweep_nf<T, 2>(mode, gw, a, grid, s);
+}
+
+void launch_finalize(const FinArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(GABE_BLOCK), 0, s, a);
+}
+
+template <typename T>
+void launch_gw(const GwArgs<T> &a, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((a.M + GABE_BLOCK - 1) / GABE_BLOCK);
+    hipLaunchKernelGGL((k_gw<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, a);
+}
+
+template <typename T>
+void launch_tt_out(const T *shat, int stage, int N, long M, double dx3, double *out, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
+    hipLaunchKernelGGL((k_tt_out<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, shat, stage, N, M, dx3, out);
+}
+
+template <typename T>
+void launch_spectra(const SpecArgs<T> &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_spectra<T>), dim3(a.numbins), dim3(GABE_BLOCK), 0, s, a);
+}
+
+template <typename T>
+void launch_effmass(const T *f0, const T *f1, int nf, int fld, double pbar0, double pbar1, gabe_model m, long n3,
+                    double *partials, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_effmass<T>), dim3(nblk), dim3(GABE_BLOCK), 0, s, f0, f1, nf, fld, pbar0, pbar1, m, n3,
+                       partials);
+}
+
+template <typename T>
+void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((n + GABE_BLOCK - 1) / GABE_BLOCK);
+    hipLaunchKernelGGL((k_add_real<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, dst, add, n);
+}
+
+#define GABE_INST(T)                                                                                          \
+    template void launch_sweep<T>(int, int, bool, const SweepArgs<T> &, dim3, hipStream_t);                   \
+    template void launch_gw<T>(const GwArgs<T> &, hipStream_t);                                               \
+    template void launch_tt_out<T>(const T *, int, int, long, double, double *, hipStream_t);                 \
+    template void launch_spectra<T>(const SpecArgs<T> &, hipStream_t);                                        \
+    template void launch_effmass<T>(const T *, const T *, int, int, double, double, gabe_model, long,        \
+                                    double *, int, hipStream_t);                                              \
+    template void launch_add_real<T>(T *, const double *, long, hipStream_t);
+GABE_INST(float)
+GABE_INST(double)

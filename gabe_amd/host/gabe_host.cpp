@@ -1,0 +1,107 @@
+/*
+ * gabe_host.cpp -- initial conditions and start-up sequence of the kept host
+ * program, on top of the C ABI (include/gabe.h).
+ */
+#include "gabe_host.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+extern "C" int gabe_host_modes_from_draws(int N, double L, double rescale_B, double d2vdf2, double H0,
+                                          const int *draws, double *fk, double *fkd)
+{
+    if (N < 4 || !draws || !fk || !fkd) return GABE_ERR_ARG;
+    const int H = N / 2 + 1;
+    const double dk2 = 4. * M_PI * M_PI / L / L;
+    long q = 0;
+    for (int i = 0; i < N; i++) {
+        const double i2 = (i < (N / 2 + 1) ? (i * i) : ((N - i) * (N - i)));
+        for (int j = 0; j < N; j++) {
+            const double j2 = (j < (N / 2 + 1) ? (j * j) : ((N - j) * (N - j)));
+            for (int k = 0; k < H; k++) {
+                const double k2 = (k * k);
+                const long m = ((long)i * N + j) * H + k;
+                if (i + j + k == 0) {          /* zero mode stays zero (g2init.cpp:95-110) */
+                    fk[2 * m] = fk[2 * m + 1] = 0.;
+                    fkd[2 * m] = fkd[2 * m + 1] = 0.;
+                    continue;
+                }
+                const double wk = std::sqrt(dk2 * (i2 + j2 + k2) + d2vdf2);
+                double r1[2], r2[2];
+                for (double *r : {r1, r2}) {   /* two (modulus, phase) draws per mode */
+                    const double u = (double)draws[q++] / (double)RAND_MAX;
+                    const double th = 2. * M_PI * (double)draws[q++] / (double)RAND_MAX;
+                    const double mag = std::sqrt(-std::log(u) / (2. * wk * L * L * L));
+                    r[0] = rescale_B * std::cos(th) * mag;
+                    r[1] = rescale_B * std::sin(th) * mag;
+                }
+                const double s2 = std::sqrt(2.);
+                fk[2 * m] = (r1[0] + r2[0]) / s2;
+                fk[2 * m + 1] = (r1[1] + r2[1]) / s2;
+                fkd[2 * m] = (r2[1] - r1[1]) * wk / s2 - H0 * fk[2 * m];
+                fkd[2 * m + 1] = (r1[0] - r2[0]) * wk / s2 - H0 * fk[2 * m + 1];
+            }
+        }
+    }
+    return GABE_OK;
+}
+
+extern "C" long gabe_host_draws_per_field(int N) { return 4L * ((long)N * N * (N / 2 + 1) - 1); }
+
+extern "C" int gabe_host_draw_modes(int N, double L, double rescale_B, double d2vdf2, double H0, double *fk,
+                                    double *fkd)
+{
+    if (N < 4) return GABE_ERR_ARG;
+    std::vector<int> draws((size_t)gabe_host_draws_per_field(N));
+    for (auto &d : draws) d = rand();
+    return gabe_host_modes_from_draws(N, L, rescale_B, d2vdf2, H0, draws.data(), fk, fkd);
+}
+
+extern "C" int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const gabe_host_init *init)
+{
+    if (!ctx || !p || !init) return GABE_ERR_ARG;
+    const int N = p->N, nf = p->nflds;
+    const long N3 = (long)N * N * N, M = (long)N * N * (N / 2 + 1);
+    int rc;
+    {   /* initfields #1 (g2model.cpp:122-125,145) */
+        std::vector<double> f((size_t)nf * N3), d((size_t)nf * N3);
+        for (int fld = 0; fld < nf; fld++)
+            for (long n = 0; n < N3; n++) {
+                f[(size_t)fld * N3 + n] = init->f0[fld];
+                d[(size_t)fld * N3 + n] = init->df0[fld];
+            }
+        if ((rc = gabe_upload_state(ctx, f.data(), d.data())) != GABE_OK) return rc;
+    }
+    if ((rc = gabe_energy(ctx, 0)) != GABE_OK) return rc;
+    gabe_scalars s;
+    if ((rc = gabe_get_scalars(ctx, &s)) != GABE_OK) return rc;
+    /* initexpansion (g2init.cpp:16-43) */
+    s.a[0] = 1.;
+    if (p->expansion_type == 0) {
+        s.adot[0] = 0.; s.a[1] = 1.; s.adot[1] = 0.;
+    } else {
+        s.adot[0] = std::sqrt(8. * M_PI / 3. * s.edrho[0]) * s.a[0];
+    }
+    if ((rc = gabe_set_scalars(ctx, &s)) != GABE_OK) return rc;
+    if (init->rand_init) {   /* initfields #2 (g2model.cpp:128-141) */
+        const double H0 = std::sqrt(8. * M_PI / 3. * s.edrho[0]);
+        /* every rand() value of every field is drawn first, in the reference's
+         * order (field by field, g2model.cpp:131-133): device and FFT library
+         * calls made in between must not be able to shift the sequence */
+        const long per = gabe_host_draws_per_field(N);
+        std::vector<int> draws((size_t)per * nf);
+        srand(init->randseed);
+        for (auto &d : draws) d = rand();
+        std::vector<double> fk(2 * M), fkd(2 * M);
+        for (int fld = 0; fld < nf; fld++) {
+            double m2;
+            if ((rc = gabe_effmass(ctx, fld, &m2)) != GABE_OK) return rc;
+            if ((rc = gabe_host_modes_from_draws(N, p->L, init->rescale_B, m2, H0, draws.data() + (size_t)fld * per,
+                                                 fk.data(), fkd.data())) != GABE_OK)
+                return rc;
+            if ((rc = gabe_add_fluctuation(ctx, fld, fk.data(), fkd.data())) != GABE_OK) return rc;
+        }
+    }
+    return gabe_energy(ctx, 0);
+}

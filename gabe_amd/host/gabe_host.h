@@ -1,0 +1,43 @@
+/*
+ * gabe_host.h -- host-side pieces of the kept GABE program that sit above the
+ * C ABI: the vacuum initial conditions and the initialisation sequence.
+ */
+#ifndef GABE_HOST_H
+#define GABE_HOST_H
+
+#include "gabe.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gabe_host_init {
+    double f0[GABE_MAX_FLDS];   /* homogeneous field values      (g2parameters.h:47) */
+    double df0[GABE_MAX_FLDS];  /* homogeneous field velocities  (g2parameters.h:48) */
+    double rescale_B;           /* IC amplitude scale            (g2parameters.h:69) */
+    unsigned randseed;          /* srand seed                    (g2parameters.h:56) */
+    int rand_init;              /* 1: add vacuum fluctuations    (g2parameters.h:71) */
+} gabe_host_init;
+
+/* Draw the half-spectrum vacuum modes of one field with libc rand() in the
+ * sequential loop order of g2init.cpp:88-172 (field_full_rand 1).  fk and fkd
+ * receive [N][N][N/2+1] (re,im) pairs.  The caller seeds rand(). */
+int gabe_host_draw_modes(int N, double L, double rescale_B, double d2vdf2, double H0, double *fk, double *fkd);
+
+/* Number of rand() values one field consumes: 4 per mode except k = 0. */
+long gabe_host_draws_per_field(int N);
+
+/* The same modes from pre-drawn rand() values (draws[gabe_host_draws_per_field(N)]). */
+int gabe_host_modes_from_draws(int N, double L, double rescale_B, double d2vdf2, double H0, const int *draws,
+                               double *fk, double *fkd);
+
+/* The start-up sequence of g2main.cpp:109-119: homogeneous fields and
+ * calcEnergy(0); initexpansion(); srand(randseed) and per field (in order)
+ * effMass -> modes -> transform-and-add; calcEnergy(0).  adot[0] keeps its
+ * homogeneous value, as in the reference. */
+int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const gabe_host_init *init);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""ctypes binding of libgabe_amd.so (include/gabe.h) with reference-shaped names.
+
+The reference drives one lattice through globals and the calls step(),
+calcEnergy(s), adf(s), fft_stresstensor(s) and specOut() (g2header.h:127-177).
+`Lattice` exposes the same operations on a device-resident lattice.  There is
+no CPU fallback: a missing or unloadable library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgabe_amd.so")
+
+GABE_OK = 0
+GABE_ERR_ARG = -1
+GABE_ERR_HIP = -2
+GABE_ERR_FFT = -3
+GABE_ERR_NONFINITE = -4
+GABE_ERR_STATE = -5
+GABE_FP64 = 0
+GABE_FP32 = 1
+GABE_NKCLASS = 5
+KERNEL_CLASSES = ("sweep_a", "finalize", "stress_fft", "gw_modes", "sweep_b")
+
+EXPORTS = (
+    "gabe_create", "gabe_destroy", "gabe_last_error", "gabe_numbins",
+    "gabe_upload_state", "gabe_download_state", "gabe_upload_gw", "gabe_download_gw",
+    "gabe_set_scalars", "gabe_get_scalars", "gabe_energy", "gabe_step", "gabe_sync",
+    "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
+    "gabe_history", "gabe_kernel_times",
+)
+
+
+class GabeParams(C.Structure):
+    _fields_ = [
+        ("N", C.c_int), ("nflds", C.c_int), ("L", C.c_double), ("dt", C.c_double),
+        ("calc_gws", C.c_int), ("expansion_type", C.c_int), ("precision", C.c_int),
+        ("m2_phi", C.c_double), ("lam", C.c_double), ("g2", C.c_double), ("m2_chi", C.c_double),
+        ("gw_spectrum_quirk", C.c_int), ("planes_per_block", C.c_int), ("reserved", C.c_int * 6),
+    ]
+
+
+class GabeScalars(C.Structure):
+    _fields_ = [
+        ("a", C.c_double * 2), ("adot", C.c_double * 2), ("edkin", C.c_double * 2),
+        ("edpot", C.c_double * 2), ("edgrad", C.c_double * 2), ("edrho", C.c_double * 2),
+        ("t", C.c_double), ("nstep", C.c_longlong), ("adot0_valid", C.c_int), ("error", C.c_int),
+    ]
+
+
+class GabeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gabe error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libgabe_amd.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    sig = {
+        "gabe_create": [C.POINTER(GabeParams), C.POINTER(vp)],
+        "gabe_destroy": [vp],
+        "gabe_numbins": [ip],
+        "gabe_upload_state": [vp, dp, dp],
+        "gabe_download_state": [vp, dp, dp],
+        "gabe_upload_gw": [vp, dp, dp],
+        "gabe_download_gw": [vp, dp, dp],
+        "gabe_set_scalars": [vp, C.POINTER(GabeScalars)],
+        "gabe_get_scalars": [vp, C.POINTER(GabeScalars)],
+        "gabe_energy": [vp, ip],
+        "gabe_step": [vp, ip],
+        "gabe_sync": [vp],
+        "gabe_spectra": [vp, dp],
+        "gabe_add_fluctuation": [vp, ip, dp, dp],
+        "gabe_effmass": [vp, ip, dp],
+        "gabe_tt_source": [vp, ip, dp],
+        "gabe_history": [vp, dp, ip],
+        "gabe_kernel_times": [vp, ip, dp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.gabe_last_error.restype = C.c_char_p
+    L.gabe_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc):
+    if rc < 0:
+        raise GabeError(rc, load().gabe_last_error().decode())
+    return rc
+
+
+def numbins(N):
+    return load().gabe_numbins(N)
+
+
+class Lattice:
+    """One device-resident lattice (a gabe_ctx).
+
+    Host arrays use the reference layouts: fields [nflds][N][N][N], tensor
+    perturbations [12][N][N][N/2+1] with planes (11,12,13,22,23,33) x (Re,Im).
+    """
+
+    def __init__(self, N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1,
+                 precision=GABE_FP64, m2_phi=1.0, lam=0.0, g2=0.0, m2_chi=0.0,
+                 gw_spectrum_quirk=1, planes_per_block=0):
+        if dt is None:
+            dt = L / N / 20.0
+        self.params = GabeParams(N, nflds, L, dt, calc_gws, expansion_type, precision,
+                                 m2_phi, lam, g2, m2_chi, gw_spectrum_quirk, planes_per_block)
+        self.N, self.nflds, self.calc_gws = N, nflds, calc_gws
+        self.M = N * N * (N // 2 + 1)
+        h = C.c_void_p()
+        _check(load().gabe_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().gabe_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state -------------------------------------------------------------
+    def upload_state(self, field, dfield):
+        f = np.ascontiguousarray(field, dtype=np.float64).reshape(self.nflds, self.N, self.N, self.N)
+        d = np.ascontiguousarray(dfield, dtype=np.float64).reshape(f.shape)
+        _check(load().gabe_upload_state(self._h, _dp(f), _dp(d)))
+
+    def download_state(self):
+        shp = (self.nflds, self.N, self.N, self.N)
+        f = np.empty(shp)
+        d = np.empty(shp)
+        _check(load().gabe_download_state(self._h, _dp(f), _dp(d)))
+        return f, d
+
+    def upload_gw(self, h=None, l=None):
+        shp = (12, self.N, self.N, self.N // 2 + 1)
+        hh = None if h is None else np.ascontiguousarray(h, dtype=np.float64).reshape(shp)
+        ll = None if l is None else np.ascontiguousarray(l, dtype=np.float64).reshape(shp)
+        _check(load().gabe_upload_gw(self._h, None if hh is None else _dp(hh), None if ll is None else _dp(ll)))
+
+    def download_gw(self):
+        shp = (12, self.N, self.N, self.N // 2 + 1)
+        h = np.empty(shp)
+        l = np.empty(shp)
+        _check(load().gabe_download_gw(self._h, _dp(h), _dp(l)))
+        return h, l
+
+    def set_scalars(self, a, adot, edkin=(0, 0), edpot=(0, 0), edgrad=(0, 0), edrho=(0, 0), t=0.0, nstep=0):
+        s = GabeScalars()
+        for k in range(2):
+            s.a[k], s.adot[k] = a[k], adot[k]
+            s.edkin[k], s.edpot[k], s.edgrad[k], s.edrho[k] = edkin[k], edpot[k], edgrad[k], edrho[k]
+        s.t, s.nstep = t, nstep
+        _check(load().gabe_set_scalars(self._h, C.byref(s)))
+
+    def get_scalars(self):
+        s = GabeScalars()
+        rc = load().gabe_get_scalars(self._h, C.byref(s))
+        out = {k: [getattr(s, k)[0], getattr(s, k)[1]] for k in ("a", "adot", "edkin", "edpot", "edgrad", "edrho")}
+        out.update(t=s.t, nstep=s.nstep, adot0_valid=s.adot0_valid, error=s.error)
+        _check(rc)
+        return out
+
+    # -- reference operations ---------------------------------------------
+    def calcEnergy(self, s=0):
+        _check(load().gabe_energy(self._h, s))
+
+    def step(self, nsteps=1):
+        _check(load().gabe_step(self._h, nsteps))
+
+    def sync(self):
+        _check(load().gabe_sync(self._h))
+
+    def specOut(self):
+        nb = numbins(self.N)
+        ncol = 2 * self.nflds + (3 if self.calc_gws else 0)
+        out = np.zeros(ncol * nb)
+        _check(load().gabe_spectra(self._h, _dp(out)))
+        return out.reshape(ncol, nb)
+
+    def add_fluctuation(self, fld, fk, fkd):
+        fk = np.ascontiguousarray(fk, dtype=np.complex128)
+        fkd = np.ascontiguousarray(fkd, dtype=np.complex128)
+        _check(load().gabe_add_fluctuation(self._h, fld, _dp(fk.view(np.float64)), _dp(fkd.view(np.float64))))
+
+    def effMass(self, fld):
+        out = C.c_double()
+        _check(load().gabe_effmass(self._h, fld, C.byref(out)))
+        return out.value
+
+    def tt_source(self, s):
+        out = np.empty((12, self.N, self.N, self.N // 2 + 1))
+        _check(load().gabe_tt_source(self._h, s, _dp(out)))
+        return out
+
+    def history(self, max_rows=1 << 16):
+        out = np.zeros((max_rows, 8))
+        n = _check(load().gabe_history(self._h, _dp(out), max_rows))
+        return out[:n]
+
+    def kernel_times(self, nsteps):
+        out = np.zeros(GABE_NKCLASS)
+        _check(load().gabe_kernel_times(self._h, nsteps, _dp(out)))
+        return dict(zip(KERNEL_CLASSES, out.tolist()))

@@ -1,0 +1,152 @@
+/*
+ * gabe.h -- C ABI of the MI355X-native lattice evolution (libgabe_amd.so).
+ *
+ * The reference program (GABE 2.x) keeps its lattice state in globals and
+ * advances it through internal C++ calls (g2header.h:52-63,127-177).  This
+ * header is the boundary that replaces those calls: the host program keeps
+ * its driver loop, initialisation and writers, and calls these entry points
+ * instead of step(), calcEnergy(), fft_stresstensor()/evolve_perts() and the
+ * computational part of specOut().
+ *
+ * Conventions
+ *   - plain C, no exceptions cross the boundary; every function returns an
+ *     int status (GABE_OK = 0, negative on error) and gabe_last_error()
+ *     returns the message of the most recent failure on the calling thread;
+ *   - the caller owns host buffers, the library owns device memory;
+ *   - one host thread per context, one HIP stream per context; calls on one
+ *     context are not re-entrant;
+ *   - gabe_step() is asynchronous (stream-ordered); every call that returns
+ *     host data synchronises the stream first;
+ *   - host arrays use the reference's slot-0 layouts: fields are
+ *     [nflds][N][N][N] doubles with the last index fastest (g2header.h:52-53);
+ *     tensor perturbations are [12][N][N][N/2+1] doubles, planes
+ *     (11,12,13,22,23,33) x (Re,Im) (g2header.h:172-174).
+ */
+#ifndef GABE_H
+#define GABE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GABE_OK 0
+#define GABE_ERR_ARG (-1)       /* invalid argument or parameter */
+#define GABE_ERR_HIP (-2)       /* HIP runtime failure (allocation, launch) */
+#define GABE_ERR_FFT (-3)       /* rocFFT plan or execution failure */
+#define GABE_ERR_NONFINITE (-4) /* energy density became NaN/Inf: unstable run */
+#define GABE_ERR_STATE (-5)     /* call not valid in the current state */
+
+#define GABE_FP64 0             /* fields stored and stepped in FP64 (oracle form) */
+#define GABE_FP32 1             /* FP64 mean + FP32 fluctuation per field */
+
+#define GABE_MAX_FLDS 2
+
+typedef struct gabe_ctx gabe_ctx;
+
+/* Runtime mirror of the g2parameters.h surface (g2parameters.h:43-100) and of
+ * the model constants used by g2model.cpp:59-108. */
+typedef struct gabe_params {
+    int N;               /* points per side, N >= 4, even           (N)            */
+    int nflds;           /* 1 or 2                                  (num_flds)     */
+    double L;            /* box side in program units               (L)            */
+    double dt;           /* time step                               (dt)           */
+    int calc_gws;        /* 1: evolve tensor perturbations          (calc_gws)     */
+    int expansion_type;  /* 0 none, 1 self-consistent, 2 user-defined            */
+    int precision;       /* GABE_FP64 or GABE_FP32                                 */
+    /* model, program units: V = m2_phi phi^2/2 + lambda phi^4/4
+     *                           + g2 phi^2 chi^2/2 + m2_chi chi^2/2
+     * (the reference default g2model.cpp:66-87 is m2_phi = 1,
+     *  g2 = gsq/mphi^2, lambda = m2_chi = 0)                                      */
+    double m2_phi, lambda, g2, m2_chi;
+    int gw_spectrum_quirk; /* 1: GW spectrum kz=0 term read from the kz=N/2 plane,
+                              as g2spectra.cpp:149-164 does; 0: read kz=0         */
+    int planes_per_block;  /* sweep tiling: lattice planes each workgroup marches
+                              through (0 = library default)                        */
+    int reserved[6];
+} gabe_params;
+
+/* The reference's expansion/energy globals (g2header.h:59-64) plus bookkeeping. */
+typedef struct gabe_scalars {
+    double a[2], adot[2], edkin[2], edpot[2], edgrad[2], edrho[2];
+    double t;             /* simulation time of the current state */
+    long long nstep;      /* steps taken since creation */
+    int adot0_valid;      /* 1: adot[0] belongs to the current state (set on write) */
+    int error;            /* nonzero once a non-finite energy density was seen */
+} gabe_scalars;
+
+/* Allocate device state for one lattice (replaces the allocations of
+ * g2main.cpp:49-99).  Fields, h and l start at zero. */
+int gabe_create(const gabe_params *params, gabe_ctx **out);
+int gabe_destroy(gabe_ctx *ctx);
+const char *gabe_last_error(void);
+
+/* Number of spectral bins, (int)(sqrt(3) N/2 + 1) (g2spectra.cpp:31). */
+int gabe_numbins(int N);
+
+/* Slot-0 field state, host layout [nflds][N][N][N] (g2main.cpp:24-26).
+ * With GABE_FP32 the upload splits each field into its FP64 mean and an FP32
+ * fluctuation; the download returns mean + fluctuation. */
+int gabe_upload_state(gabe_ctx *ctx, const double *field, const double *dfield);
+int gabe_download_state(gabe_ctx *ctx, double *field, double *dfield);
+
+/* Slot-0 tensor perturbations h and l = dh/dt, layout [12][N][N][N/2+1]
+ * (g2main.cpp:30-35).  Either pointer may be NULL. */
+int gabe_upload_gw(gabe_ctx *ctx, const double *h, const double *l);
+int gabe_download_gw(gabe_ctx *ctx, double *h, double *l);
+
+/* Expansion and energy globals.  gabe_set_scalars marks adot[0] as belonging
+ * to the current state (the first step then uses it, as step() does after
+ * initexpansion(), g2init.cpp:26-29).  gabe_get_scalars returns the reference's
+ * post-step values: a[0], and edkin/edpot/edgrad/edrho[0] and adot[0] = adf(0)
+ * of the current state (g2functions.cpp:268-270), computing them if stale. */
+int gabe_set_scalars(gabe_ctx *ctx, const gabe_scalars *s);
+int gabe_get_scalars(gabe_ctx *ctx, gabe_scalars *s);
+
+/* calcEnergy(0) (g2functions.cpp:168-174): energies of the current state into
+ * slot 0; adot[0] is left unchanged.  Only s = 0 is valid between steps. */
+int gabe_energy(gabe_ctx *ctx, int s);
+
+/* nsteps midpoint-RK2 steps of fields, tensor modes and a(t)
+ * (step(), g2functions.cpp:201-304).  Asynchronous. */
+int gabe_step(gabe_ctx *ctx, int nsteps);
+
+/* Wait for all queued work; reports GABE_ERR_NONFINITE if the energy density
+ * became non-finite. */
+int gabe_sync(gabe_ctx *ctx);
+
+/* Computational part of specOut (g2spectra.cpp:23-197).  out receives
+ * [ncols][numbins] doubles: P and k^3 P/(2 pi^2) per field, then, with GW,
+ * the GW power, the Omega_gw column and the trailing k column, i.e.
+ * ncols = 2*nflds (+3 with GW).  The host keeps the writer. */
+int gabe_spectra(gabe_ctx *ctx, double *out);
+
+/* randInit's transform-and-add (g2init.cpp:174-187): fk and fkd are the
+ * half-spectrum modes [N][N][N/2+1] as (re,im) pairs, as drawn by the host's
+ * sequential rand() loop (g2init.cpp:88-172).  The kz=0 and kz=N/2 planes are
+ * used by their Hermitian part.  f += c2r(fk), fdot += c2r(fkd), unnormalised. */
+int gabe_add_fluctuation(gabe_ctx *ctx, int fld, const double *fk, const double *fkd);
+
+/* effMass(0, fld) (g2model.cpp:89-108): lattice mean of d2V/dfld2. */
+int gabe_effmass(gabe_ctx *ctx, int fld, double *out);
+
+/* The TT-projected source T_gw[s] of the current state as fft_stresstensor(s)
+ * computes it (g2evolutionpert.cpp:135-339): s = 0 uses field[0], s = 1 the
+ * midpoint field[0] + dt/2 dfield[0].  out is [12][N][N][N/2+1]. */
+int gabe_tt_source(gabe_ctx *ctx, int s, double *out);
+
+/* Per-step scalar history recorded on the device at each step's start:
+ * rows of (t, a, adot, edkin, edpot, edgrad, edrho, step).  Returns the
+ * number of rows written to out (at most max_rows, most recent last). */
+int gabe_history(gabe_ctx *ctx, double *out, int max_rows);
+
+/* Timing hook: runs nsteps steps with HIP events around every kernel on the
+ * context's stream and writes the mean duration in ms of each kernel class to
+ * ms_out[0..GABE_NKCLASS).  Classes: sweep A, finalize, stress FFT, GW modes,
+ * sweep B. */
+#define GABE_NKCLASS 5
+int gabe_kernel_times(gabe_ctx *ctx, int nsteps, double *ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,199 @@
+"""Device path vs the CPU FP64 oracle on the same seeded initial states.
+
+Tolerances:
+  FP64 device vs oracle: fields and GW modes within 1e-9 of the fluctuation
+  amplitude, a/adot within 1e-13 relative, energies within 1e-12 of edrho
+  (only summation order differs);
+  FP32 form vs FP64 device: fields within 1e-4 of the fluctuation amplitude,
+  GW modes within 1e-3 of their amplitude, a/adot within 1e-10 relative,
+  over the early window tested here.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import (GABE_FP32, GABE_FP64, configs, fluct_err, lattice_from_oracle, po, rel_err,
+                           seeded_oracle)
+
+pytestmark = pytest.mark.gpu
+
+SC_KEYS = ("a", "adot", "edkin", "edpot", "edgrad", "edrho")
+
+
+def check_scalars(got, orc, rtol_a=1e-13, rtol_e=1e-12):
+    for k in ("a", "adot"):
+        assert abs(got[k][0] - orc[k][0]) <= rtol_a * abs(orc[k][0]), (k, got[k][0], orc[k][0])
+    scale = abs(orc["edrho"][0])
+    for k in ("edkin", "edpot", "edgrad", "edrho"):
+        assert abs(got[k][0] - orc[k][0]) <= rtol_e * scale, (k, got[k][0], orc[k][0])
+
+
+@pytest.mark.parametrize("N,gw", [(16, 1), (24, 1), (32, 1), (32, 0)])
+def test_fp64_steps_match_oracle(N, gw):
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=gw)
+    o = seeded_oracle(cfg, amp=2e-3)
+    lat = lattice_from_oracle(o, cfg, GABE_FP64)
+    for block in (1, 4, 15):
+        o.step(block)
+        lat.step(block)
+        f, d = lat.download_state()
+        assert fluct_err(f, o.field[0]) < 1e-9
+        assert fluct_err(d, o.dfield[0]) < 1e-9
+        check_scalars(lat.get_scalars(), o.scalars())
+        if gw:
+            h, l = lat.download_gw()
+            assert rel_err(h, o.h[0]) < 1e-9
+            assert rel_err(l, o.l[0]) < 1e-9
+
+
+def test_fp64_reference_default_model():
+    cfg = configs.reference_default_config(N=16, calc_gws=1)
+    o = seeded_oracle(cfg, amp=1e-2)
+    lat = lattice_from_oracle(o, cfg, GABE_FP64)
+    o.step(10)
+    lat.step(10)
+    f, d = lat.download_state()
+    assert fluct_err(f, o.field[0]) < 1e-9
+    assert fluct_err(d, o.dfield[0]) < 1e-9
+    check_scalars(lat.get_scalars(), o.scalars())
+    h, l = lat.download_gw()
+    assert rel_err(l, o.l[0]) < 1e-9
+
+
+@pytest.mark.parametrize("exp_type,nflds", [(0, 2), (2, 2), (1, 1)])
+def test_fp64_expansion_types_and_one_field(exp_type, nflds):
+    cfg = configs.lambda_phi4_config(N=16, calc_gws=0)
+    cfg.expansion_type = exp_type
+    if nflds == 1:
+        cfg.nflds, cfg.g2, cfg.f0, cfg.df0 = 1, 0.0, cfg.f0[:1], cfg.df0[:1]
+    if exp_type == 2:
+        # type 2 is user-defined: g2init.cpp:274-283 leaves a(0) unset; start it
+        # as type 1 does (a = 1, adot = adf(0)), then evolve with the type-2 step
+        cfg.expansion_type = 1
+        o = seeded_oracle(cfg, amp=2e-3)
+        o.st.p.expansion_type = 2
+        cfg.expansion_type = 2
+    else:
+        o = seeded_oracle(cfg, amp=2e-3)
+    lat = lattice_from_oracle(o, cfg, GABE_FP64)
+    o.step(8)
+    lat.step(8)
+    f, d = lat.download_state()
+    assert fluct_err(f, o.field[0]) < 1e-9
+    assert fluct_err(d, o.dfield[0]) < 1e-9
+    check_scalars(lat.get_scalars(), o.scalars())
+
+
+@pytest.mark.parametrize("s", [0, 1])
+def test_tt_source_matches_oracle(s):
+    cfg = configs.lambda_phi4_config(N=16, calc_gws=1)
+    o = seeded_oracle(cfg, amp=2e-3)
+    lat = lattice_from_oracle(o, cfg, GABE_FP64)
+    if s == 1:
+        o.field[1] = o.field[0] + 0.5 * cfg.dt * o.dfield[0]
+    o.fft_stresstensor(s)
+    T = lat.tt_source(s)
+    assert rel_err(T, o.T_gw[s]) < 1e-11
+
+
+@pytest.mark.parametrize("N,quirk", [(16, 1), (16, 0), (24, 1)])
+def test_spectra_match_oracle(N, quirk):
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+    o = seeded_oracle(cfg, amp=2e-3)
+    lat = lattice_from_oracle(o, cfg, GABE_FP64, gw_spectrum_quirk=quirk)
+    o.step(6)
+    lat.step(6)
+    got = lat.specOut()
+    ref = o.specout(gw_quirk=quirk)
+    for col in range(ref.shape[0]):
+        scale = np.abs(ref[col]).max()
+        assert np.abs(got[col] - ref[col]).max() <= 1e-10 * scale, col
+
+
+def test_effmass_and_fluctuation_add_match_oracle():
+    cfg = configs.lambda_phi4_config(N=16, calc_gws=0)
+    cfg.rescale_B = 1e-3
+    o = seeded_oracle(cfg)  # reference initialisation on the CPU
+    # the same modes, drawn by the oracle before any device call
+    o2 = po.Oracle(cfg.N, nflds=2, L=cfg.L, dt=cfg.dt, m2_phi=cfg.m2_phi, lam=cfg.lam, g2=cfg.g2,
+                   rescale_B=cfg.rescale_B)
+    o2.initfields_homog(cfg.f0, cfg.df0)
+    o2.initexpansion()
+    po.srand(cfg.randseed)
+    masses, modes = [], []
+    for fld in range(2):
+        masses.append(o2.effMass(0, fld))
+        modes.append(o2.randinit(fld, masses[-1], want_modes=True))
+    lat = lattice_from_oracle(o2, cfg, GABE_FP64)
+    lat.upload_state(np.broadcast_to(np.array(cfg.f0)[:, None, None, None], o2.field[0].shape),
+                     np.broadcast_to(np.array(cfg.df0)[:, None, None, None], o2.field[0].shape))
+    for fld in range(2):
+        m_dev = lat.effMass(fld)
+        assert abs(m_dev - masses[fld]) <= 1e-13 * abs(masses[fld])
+        lat.add_fluctuation(fld, *modes[fld])
+    f, d = lat.download_state()
+    assert fluct_err(f, o.field[0]) < 1e-11
+    assert fluct_err(d, o.dfield[0]) < 1e-11
+
+
+@pytest.mark.parametrize("precision", [GABE_FP64, GABE_FP32])
+def test_host_initialize_matches_reference_init(precision):
+    """gabe_host_initialize (the kept host's start-up) vs the oracle's
+    g2main.cpp:109-119 sequence: same fields, energies and adot[0]."""
+    from gabe_amd import host
+    cfg = configs.lambda_phi4_config(N=24, calc_gws=1)
+    cfg.rescale_B = 1e-3
+    o = seeded_oracle(cfg)
+    lat = host.initialized_lattice(cfg, precision=precision)
+    f, d = lat.download_state()
+    tol = 1e-11 if precision == GABE_FP64 else 1e-6
+    assert fluct_err(f, o.field[0]) < tol
+    assert fluct_err(d, o.dfield[0]) < tol
+    got, ref = lat.get_scalars(), o.scalars()
+    assert abs(got["adot"][0] - ref["adot"][0]) <= 1e-13 * ref["adot"][0]   # homogeneous adot (Q4)
+    # FP32 stores the fluctuation velocity in FP32: rho carries ~1e-12 of its rounding
+    etol = 1e-12 if precision == GABE_FP64 else 1e-10
+    assert abs(got["edrho"][0] - ref["edrho"][0]) <= etol * ref["edrho"][0]
+
+
+@pytest.mark.parametrize("N", [32, 48])
+def test_fp32_form_tracks_fp64(N):
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+    o = seeded_oracle(cfg)  # physical amplitude: fluctuations ~1e-6 of the mean
+    l64 = lattice_from_oracle(o, cfg, GABE_FP64)
+    l32 = lattice_from_oracle(o, cfg, GABE_FP32)
+    l64.step(40)
+    l32.step(40)
+    f64, d64 = l64.download_state()
+    f32, d32 = l32.download_state()
+    assert fluct_err(f32, f64) < 1e-4
+    assert fluct_err(d32, d64) < 1e-4
+    s64, s32 = l64.get_scalars(), l32.get_scalars()
+    for k in ("a", "adot"):
+        assert abs(s32[k][0] - s64[k][0]) <= 1e-10 * abs(s64[k][0])
+    h64, ll64 = l64.download_gw()
+    h32, ll32 = l32.download_gw()
+    assert rel_err(ll32, ll64) < 1e-3
+    sp64, sp32 = l64.specOut(), l32.specOut()
+    for col in range(sp64.shape[0]):
+        # bins below 1e-9 of the column maximum are roundoff (e.g. the DC bin of
+        # a zero-mean field) and carry no physics
+        nz = np.abs(sp64[col]) > 1e-9 * np.abs(sp64[col]).max()
+        assert np.max(np.abs(sp32[col][nz] / sp64[col][nz] - 1)) < 1e-3, col
+
+
+def test_history_and_deterministic_replay():
+    cfg = configs.lambda_phi4_config(N=16, calc_gws=1)
+    o = seeded_oracle(cfg, amp=2e-3)
+    a = lattice_from_oracle(o, cfg, GABE_FP32)
+    b = lattice_from_oracle(o, cfg, GABE_FP32)
+    a.step(5)
+    a.get_scalars()   # forces an energy pass between steps: must not perturb the trajectory
+    a.step(5)
+    b.step(10)
+    fa, da = a.download_state()
+    fb, db = b.download_state()
+    assert np.array_equal(fa, fb) and np.array_equal(da, db)
+    ha, hb = a.history(), b.history()
+    assert ha.shape == (10, 8)
+    assert np.array_equal(ha, hb)
+    assert np.all(np.diff(ha[:, 7]) == 1)
