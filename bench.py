@@ -1,0 +1,174 @@
+"""Throughput of the lattice step on one MI355X.
+
+Workload: a 256^3 periodic lattice with two scalar fields (V = lambda phi^4/4
++ g^2 phi^2 chi^2/2, g^2/lambda = 100, L = 20, dt = dx/20) and the six
+tensor-perturbation components, midpoint RK2 with self-consistent expansion.
+Initial state: homogeneous phi0 = 0.342 with slow-roll velocity plus the seeded
+vacuum fluctuations, produced by the kept host's start-up sequence.
+
+value  = site-updates/s of the FP32 form (FP32 fluctuations, FP64 background,
+         a(t) and reductions) = N^3 x steps / seconds, inputs resident in HBM.
+Also reported: the same for the FP64 form, per-kernel mean durations (HIP
+events on the lattice's stream), the roofline of the dominant kernel, and the
+CPU oracle timed on the host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "lattice site-updates/s (RK2, 2 fields+GW) at 256³ FP32 vs FP64; % HBM peak"
+
+
+def alg_bytes_per_site(s, N):
+    """Compulsory HBM bytes per site-update of each kernel class (s = bytes per real)."""
+    r = (N // 2 + 1) / N
+    return {
+        "sweep_a": 16 * s,                     # read 4 state arrays, write 12 stress arrays
+        "finalize": 0.0,
+        "stress_fft": 12 * s + 24 * s * r,     # 12 R2C: read reals, write half-spectra
+        "gw_modes": 72 * s * r,                # read S0,S1,h,l, write h,l (complex, per mode)
+        "sweep_b": 8 * s,                      # read 4 arrays, write 4 arrays
+    }
+
+
+def read_traffic(kernel, N):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{N}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def run_device(cfg, precision, steps, warmup, profile_steps):
+    from gabe_amd import host
+    lat = host.initialized_lattice(cfg, precision=precision)
+    lat.step(warmup)
+    lat.sync()
+    t0 = time.perf_counter()
+    lat.step(steps)
+    lat.sync()
+    t1 = time.perf_counter()
+    ktimes = lat.kernel_times(profile_steps) if profile_steps else {}
+    sc = lat.get_scalars()
+    state = lat.download_state() if precision == 1 else None
+    lat.close()
+    return t1 - t0, ktimes, sc, state
+
+
+def cpu_baseline(cfg, state, nsteps=2):
+    """The CPU FP64 oracle, OpenMP over the outer index, on this host's cores."""
+    from oracle import pyoracle as po
+    threads = min(16, os.cpu_count() or 1)
+    o = po.Oracle(cfg.N, nflds=cfg.nflds, L=cfg.L, dt=cfg.dt, calc_gws=cfg.calc_gws,
+                  expansion_type=cfg.expansion_type, m2_phi=cfg.m2_phi, lam=cfg.lam, g2=cfg.g2,
+                  m2_chi=cfg.m2_chi, rescale_B=cfg.rescale_B, nthreads=threads)
+    f, d = state
+    o.field[0] = f
+    o.dfield[0] = d
+    o.set_scalar("a", 0, 1.0)
+    o.calcEnergy(0)
+    o.set_scalar("adot", 0, o.adf(0))
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    t1 = time.perf_counter()
+    return {
+        "value": cfg.N ** 3 * nsteps / (t1 - t0),
+        "unit": "site-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{nsteps} RK2 steps of the same {cfg.N}^3 2-field+GW lattice in FP64 "
+                  f"(oracle restatement, OpenMP over i, {threads} threads), {t1 - t0:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs every lattice on a single MI355X "
+              "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from gabe_amd import configs
+    from gabe_amd.lattice import GABE_FP32, GABE_FP64
+
+    N = args.N
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+    sites = N ** 3
+
+    print(f"[bench] FP32 {N}^3: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+    dt32, k32, sc32, state = run_device(cfg, GABE_FP32, args.steps, args.warmup, args.profile_steps)
+    rate32 = sites * args.steps / dt32
+    fp64 = None
+    if not args.no_fp64:
+        print(f"[bench] FP64 {N}^3", file=sys.stderr, flush=True)
+        dt64, k64, sc64, _ = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps)
+        fp64 = {"value": sites * args.steps / dt64, "ms_per_step": 1e3 * dt64 / args.steps,
+                "kernels_ms": k64,
+                "step_hbm_frac": sum(alg_bytes_per_site(8, N).values()) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9)}
+
+    ab = alg_bytes_per_site(4, N)
+    dom = max((k for k in k32 if k != "finalize"), key=lambda k: k32[k])
+    per_launch = ab[dom] * sites
+    achieved = per_launch / (k32[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(dom, N), "kernel": dom,
+                "alg_bytes_per_launch": per_launch}
+    kernels = {k: {"ms": v, "alg_GBs": (ab[k] * sites / (v * 1e-3) / 1e9) if v > 0 and ab[k] else None}
+               for k, v in k32.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(cfg, state)
+
+    out = {
+        "metric": METRIC,
+        "value": rate32,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seeded vacuum fluctuations (reference IC recipe, randseed 44463132) on the homogeneous "
+                "slow-roll background",
+        "config": {"workload": f"{N}^3 lattice, 2 scalar fields + 6 GW tensor components, midpoint RK2, "
+                               "self-consistent expansion, FP32 fields / FP64 background, a(t) and sums",
+                   "N": N, "L": cfg.L, "dt": cfg.dt, "model": "lambda phi^4/4 + g^2 phi^2 chi^2/2, g^2/lambda=100"},
+        "roofline": roofline,
+        "step_hbm_frac": sum(ab.values()) * rate32 / (HBM_PEAK_GBS * 1e9),
+        "kernels": kernels,
+        "fp64": fp64,
+        "cpu_baseline": cpu,
+        "final_a": sc32["a"][0],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
