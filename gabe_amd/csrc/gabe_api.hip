@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "gabe.h"
+#include "gabe_fft.h"
 #include "gabe_internal.h"
 
 namespace {
@@ -98,6 +99,8 @@ struct gabe_ctx {
     int hist_cap = 0;
     double *spec_out = nullptr;
     FftPlan fft_stress, fft_spec, fft_c2r;
+    bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
+    void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
     gabe_model model{};
@@ -114,12 +117,13 @@ int ensure_rocfft()
     return GABE_OK;
 }
 
-int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch)
+int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch, int rank = 3)
 {
     CHK(ensure_rocfft());
     size_t len[3] = {(size_t)c->N, (size_t)c->N, (size_t)c->N};
     FFTCHK(rocfft_plan_create(&P.plan, rocfft_placement_notinplace, type,
-                              dbl ? rocfft_precision_double : rocfft_precision_single, 3, len, batch, nullptr));
+                              dbl ? rocfft_precision_double : rocfft_precision_single, (size_t)rank, len, batch,
+                              nullptr));
     FFTCHK(rocfft_plan_get_work_buffer_size(P.plan, &P.work));
     FFTCHK(rocfft_execution_info_create(&P.info));
     FFTCHK(rocfft_execution_info_set_stream(P.info, c->stream));
@@ -216,6 +220,23 @@ int launch_gw_ctx(gabe_ctx *c)
     return GABE_OK;
 }
 
+/* R2C of the 12 stress arrays into shat [12][N][N][N/2+1] */
+int stress_fft(gabe_ctx *c)
+{
+    if (!c->colfft) return run_plan(c, c->fft_stress, c->stress, c->shat);
+    CHK(run_plan(c, c->fft_stress, c->stress, c->shat));            /* rows along k */
+    const long KP = c->N / 2 + 1;
+    if (c->fp32) {
+        launch_colfft<float>(c->shat, c->N, KP, 12L * c->N, c->twiddle, c->stream);       /* along j */
+        launch_colfft<float>(c->shat, c->N, c->N * KP, 12, c->twiddle, c->stream);        /* along i */
+    } else {
+        launch_colfft<double>(c->shat, c->N, KP, 12L * c->N, c->twiddle, c->stream);
+        launch_colfft<double>(c->shat, c->N, c->N * KP, 12, c->twiddle, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
 /* the kernels of one step; ev (optional) receives 2 events per kernel class */
 int one_step(gabe_ctx *c, hipEvent_t *ev)
 {
@@ -230,7 +251,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     CHK(launch_finalize_ctx(c, SW_A, 0));
     CHK(rec(3));
     CHK(rec(4));
-    if (c->gw) CHK(run_plan(c, c->fft_stress, c->stress, c->shat));
+    if (c->gw) CHK(stress_fft(c));
     CHK(rec(5));
     CHK(rec(6));
     if (c->gw) CHK(launch_gw_ctx(c));
@@ -339,7 +360,26 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
     const int nb = gabe_numbins(c->N);
     HIPCHK(hipMalloc(&c->spec_out, sizeof(double) * nb * (GABE_MAXF + 2)));
 
-    if (c->alloc_gw) CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12));
+    if (c->alloc_gw) {
+        c->colfft = gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
+        if (c->colfft) {
+            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12 * (size_t)c->N2, 1));
+            std::vector<double> tw(2 * c->N);
+            for (int e = 0; e < c->N; e++) {
+                tw[2 * e] = gabe_cos2pi_frac(e, c->N);
+                tw[2 * e + 1] = -gabe_sin2pi_frac(e, c->N);
+            }
+            HIPCHK(hipMalloc(&c->twiddle, 2 * c->esz * c->N));
+            if (c->fp32) {
+                std::vector<float> twf(tw.begin(), tw.end());
+                HIPCHK(hipMemcpy(c->twiddle, twf.data(), 2 * c->esz * c->N, hipMemcpyHostToDevice));
+            } else {
+                HIPCHK(hipMemcpy(c->twiddle, tw.data(), 2 * c->esz * c->N, hipMemcpyHostToDevice));
+            }
+        } else {
+            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12));
+        }
+    }
     CHK(make_plan(c, c->fft_spec, rocfft_transform_type_real_forward, !c->fp32, c->nf));
     return GABE_OK;
 }
@@ -350,7 +390,7 @@ void destroy_impl(gabe_ctx *c)
     destroy_plan(c->fft_spec);
     destroy_plan(c->fft_c2r);
     void *bufs[] = {c->fbuf[0], c->fbuf[1], c->vbuf[0], c->vbuf[1], c->stress, c->shat, c->h, c->l,
-                    c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work};
+                    c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work, c->twiddle};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -724,7 +764,7 @@ int gabe_tt_source(gabe_ctx *c, int s, double *out)
     double *dout = nullptr;
     HIPCHK(hipMalloc(&dout, sizeof(double) * 12 * c->M));
     int rc = launch_sweep_ctx(c, SW_A, true);
-    if (rc == GABE_OK) rc = run_plan(c, c->fft_stress, c->stress, c->shat);
+    if (rc == GABE_OK) rc = stress_fft(c);
     if (rc == GABE_OK) {
         const double dx3 = c->dx * c->dx * c->dx;
         if (c->fp32) launch_tt_out<float>((const float *)c->shat, s, c->N, c->M, dx3, dout, c->stream);

@@ -243,37 +243,51 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
  * expansion update of g2functions.cpp:250-252,268-270 and the background
  * update of the FP32 form.  Deterministic: the same partials give the same
  * bits on every run. */
+template <int NF>
 __global__ __launch_bounds__(GABE_BLOCK) void k_finalize(FinArgs F)
 {
-    const int ns = gabe_nsums(F.nf);
-    double s[GABE_MAXSUMS], c[GABE_MAXSUMS];
-    for (int q = 0; q < GABE_MAXSUMS; q++) { s[q] = 0.; c[q] = 0.; }
+    constexpr int NS = gabe_nsums(NF);
+    double s[NS], c[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) { s[q] = 0.; c[q] = 0.; }
     for (int b = threadIdx.x; b < F.nblk; b += GABE_BLOCK) {
-        for (int q = 0; q < ns; q++) {
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
             double t, e;
             two_sum(s[q], F.partials[(long)q * F.nblk + b], t, e);
             s[q] = t;
             c[q] += e;
         }
     }
-    __shared__ double sh_s[GABE_BLOCK], sh_c[GABE_BLOCK];
+    /* butterfly across the wavefront, then the four wavefronts in order */
+    __shared__ double sh_s[GABE_BLOCK / 64][NS], sh_c[GABE_BLOCK / 64][NS];
     __shared__ double tot[GABE_MAXSUMS];
-    for (int q = 0; q < ns; q++) {
-        sh_s[threadIdx.x] = s[q];
-        sh_c[threadIdx.x] = c[q];
-        __syncthreads();
-        for (int w = GABE_BLOCK / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) {
-                double t, e;
-                two_sum(sh_s[threadIdx.x], sh_s[threadIdx.x + w], t, e);
-                sh_s[threadIdx.x] = t;
-                sh_c[threadIdx.x] = sh_c[threadIdx.x] + sh_c[threadIdx.x + w] + e;
-            }
-            __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double so = __shfl_xor(s[q], off), co = __shfl_xor(c[q], off);
+            double t, e;
+            two_sum(s[q], so, t, e);
+            s[q] = t;
+            c[q] = c[q] + co + e;
         }
-        if (threadIdx.x == 0) tot[q] = sh_s[0] + sh_c[0];
-        __syncthreads();
+        if (lane == 0) { sh_s[wv][q] = s[q]; sh_c[wv][q] = c[q]; }
     }
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        const int q = threadIdx.x;
+        double ss = sh_s[0][q], cc = sh_c[0][q];
+        for (int w = 1; w < GABE_BLOCK / 64; w++) {
+            double t, e;
+            two_sum(ss, sh_s[w][q], t, e);
+            ss = t;
+            cc = cc + sh_c[w][q] + e;
+        }
+        tot[q] = ss + cc;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
 
     DevScalars *sc = F.sc;
@@ -601,7 +615,8 @@ void launch_sweep(int mode, int nf, bool gw, const SweepArgs<T> &a, dim3 grid, h
 
 void launch_finalize(const FinArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(GABE_BLOCK), 0, s, a);
+    if (a.nf == 1) hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(GABE_BLOCK), 0, s, a);
+    else hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(GABE_BLOCK), 0, s, a);
 }
 
 template <typename T>

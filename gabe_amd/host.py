@@ -26,6 +26,8 @@ def load():
         L = C.CDLL(LIB_PATH)
         L.gabe_host_initialize.argtypes = [C.c_void_p, C.POINTER(_lat.GabeParams), C.POINTER(HostInit)]
         L.gabe_host_initialize.restype = C.c_int
+        L.gabe_host_friedmann_residual.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+        L.gabe_host_friedmann_residual.restype = C.c_double
         _lib = L
     return _lib
 
@@ -40,6 +42,16 @@ def initialize(lat, cfg, rand_init=1):
     init.randseed = cfg.randseed
     init.rand_init = rand_init
     _lat._check(load().gabe_host_initialize(lat._h, C.byref(lat.params), C.byref(init)))
+
+
+def friedmann_residual(hist):
+    """R(t) = |rho_dot + 3H(rho + p)| / (3 H rho) per history row (0 at the ends)."""
+    import numpy as np
+    h = np.ascontiguousarray(hist, dtype=np.float64)
+    out = np.zeros(h.shape[0])
+    worst = load().gabe_host_friedmann_residual(h.ctypes.data_as(C.POINTER(C.c_double)), h.shape[0],
+                                                out.ctypes.data_as(C.POINTER(C.c_double)))
+    return worst, out
 
 
 def initialized_lattice(cfg, precision=_lat.GABE_FP32, **kw):

@@ -105,3 +105,20 @@ extern "C" int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const g
     }
     return gabe_energy(ctx, 0);
 }
+
+extern "C" double gabe_host_friedmann_residual(const double *hist, int nrows, double *out)
+{
+    double worst = 0.;
+    for (int n = 0; n < nrows; n++) {
+        if (out) out[n] = 0.;
+        if (n == 0 || n == nrows - 1) continue;
+        const double *r = hist + 8L * n, *rm = hist + 8L * (n - 1), *rp = hist + 8L * (n + 1);
+        const double H = r[2] / r[1];
+        const double rho = r[6], p = r[3] - r[5] / 3. - r[4];
+        const double rhodot = (rp[6] - rm[6]) / (rp[0] - rm[0]);
+        const double R = std::fabs(rhodot + 3. * H * (rho + p)) / (3. * H * rho);
+        if (out) out[n] = R;
+        if (R > worst) worst = R;
+    }
+    return worst;
+}

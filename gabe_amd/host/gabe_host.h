@@ -37,6 +37,14 @@ int gabe_host_modes_from_draws(int N, double L, double rescale_B, double d2vdf2,
  * homogeneous value, as in the reference. */
 int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const gabe_host_init *init);
 
+/* Friedmann-constraint residual from a per-step scalar history (rows of
+ * gabe_history: t, a, adot, edkin, edpot, edgrad, edrho, step).  For each
+ * interior row n: R = |rho_dot + 3 H (rho + p)| / (3 H rho), with
+ * p = edkin - edgrad/3 - edpot, H = adot/a and rho_dot the central difference
+ * (rho[n+1] - rho[n-1]) / (t[n+1] - t[n-1]).  out[n] = R (0 for the first and
+ * last rows).  Returns the maximum over interior rows. */
+double gabe_host_friedmann_residual(const double *hist, int nrows, double *out);
+
 #ifdef __cplusplus
 }
 #endif
