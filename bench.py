@@ -53,9 +53,9 @@ def read_traffic(kernel, N):
         return None
 
 
-def run_device(cfg, precision, steps, warmup, profile_steps):
+def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
     from gabe_amd import host
-    lat = host.initialized_lattice(cfg, precision=precision)
+    lat = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb)
     lat.step(warmup)
     lat.sync()
     t0 = time.perf_counter()
@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ppb", type=int, default=0, help="lattice planes per sweep workgroup (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs every lattice on a single MI355X "
@@ -118,23 +119,25 @@ def main():
     sites = N ** 3
 
     print(f"[bench] FP32 {N}^3: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-    dt32, k32, sc32, state = run_device(cfg, GABE_FP32, args.steps, args.warmup, args.profile_steps)
+    dt32, k32, sc32, state = run_device(cfg, GABE_FP32, args.steps, args.warmup, args.profile_steps, args.ppb)
     rate32 = sites * args.steps / dt32
     fp64 = None
     if not args.no_fp64:
         print(f"[bench] FP64 {N}^3", file=sys.stderr, flush=True)
-        dt64, k64, sc64, _ = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps)
+        dt64, k64, sc64, _ = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps, args.ppb)
         fp64 = {"value": sites * args.steps / dt64, "ms_per_step": 1e3 * dt64 / args.steps,
                 "kernels_ms": k64,
                 "step_hbm_frac": sum(alg_bytes_per_site(8, N).values()) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9)}
 
     ab = alg_bytes_per_site(4, N)
-    dom = max((k for k in k32 if k != "finalize"), key=lambda k: k32[k])
-    per_launch = ab[dom] * sites
-    achieved = per_launch / (k32[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(dom, N), "kernel": dom,
-                "alg_bytes_per_launch": per_launch}
+    roofline = None
+    if k32:
+        dom = max((k for k in k32 if k != "finalize"), key=lambda k: k32[k])
+        per_launch = ab[dom] * sites
+        achieved = per_launch / (k32[dom] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(dom, N), "kernel": dom,
+                    "alg_bytes_per_launch": per_launch}
     kernels = {k: {"ms": v, "alg_GBs": (ab[k] * sites / (v * 1e-3) / 1e9) if v > 0 and ab[k] else None}
                for k, v in k32.items()}
 

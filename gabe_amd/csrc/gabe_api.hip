@@ -100,6 +100,8 @@ struct gabe_ctx {
     double *spec_out = nullptr;
     FftPlan fft_stress, fft_spec, fft_c2r;
     bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
+    long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
+    int fft_cw = 16;             /* column tile width of the j/i transforms */
     void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
@@ -117,13 +119,24 @@ int ensure_rocfft()
     return GABE_OK;
 }
 
-int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch, int rank = 3)
+int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch, int rank = 3,
+              size_t out_distance = 0)
 {
     CHK(ensure_rocfft());
     size_t len[3] = {(size_t)c->N, (size_t)c->N, (size_t)c->N};
-    FFTCHK(rocfft_plan_create(&P.plan, rocfft_placement_notinplace, type,
-                              dbl ? rocfft_precision_double : rocfft_precision_single, (size_t)rank, len, batch,
-                              nullptr));
+    rocfft_plan_description desc = nullptr;
+    if (out_distance) {   /* rank-1 rows with a padded output row stride */
+        FFTCHK(rocfft_plan_description_create(&desc));
+        size_t in_stride = 1, out_stride = 1;
+        FFTCHK(rocfft_plan_description_set_data_layout(desc, rocfft_array_type_real,
+                                                       rocfft_array_type_hermitian_interleaved, nullptr, nullptr, 1,
+                                                       &in_stride, (size_t)c->N, 1, &out_stride, out_distance));
+    }
+    const rocfft_status st = rocfft_plan_create(&P.plan, rocfft_placement_notinplace, type,
+                                                dbl ? rocfft_precision_double : rocfft_precision_single,
+                                                (size_t)rank, len, batch, desc);
+    if (desc) rocfft_plan_description_destroy(desc);
+    FFTCHK(st);
     FFTCHK(rocfft_plan_get_work_buffer_size(P.plan, &P.work));
     FFTCHK(rocfft_execution_info_create(&P.info));
     FFTCHK(rocfft_execution_info_set_stream(P.info, c->stream));
@@ -210,28 +223,30 @@ int launch_gw_ctx(gabe_ctx *c)
 {
     const double dx3 = c->dx * c->dx * c->dx;
     if (c->fp32) {
-        GwArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, c->N, c->M, c->p.L, c->p.dt, dx3};
+        GwArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3};
         launch_gw<float>(g, c->stream);
     } else {
-        GwArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss, c->N, c->M, c->p.L, c->p.dt, dx3};
+        GwArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3};
         launch_gw<double>(g, c->stream);
     }
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
 
-/* R2C of the 12 stress arrays into shat [12][N][N][N/2+1] */
+/* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid) */
 int stress_fft(gabe_ctx *c)
 {
     if (!c->colfft) return run_plan(c, c->fft_stress, c->stress, c->shat);
     CHK(run_plan(c, c->fft_stress, c->stress, c->shat));            /* rows along k */
-    const long KP = c->N / 2 + 1;
+    const long N = c->N, KP = N / 2 + 1, kp = c->kpad;
+    ColFftView vj{c->shat, kp, N * kp, 0, (int)KP, 1, 12 * N};        /* along j */
+    ColFftView vi{c->shat, N * kp, N * N * kp, kp, (int)KP, (int)N, 12}; /* along i */
     if (c->fp32) {
-        launch_colfft<float>(c->shat, c->N, KP, 12L * c->N, c->twiddle, c->stream);       /* along j */
-        launch_colfft<float>(c->shat, c->N, c->N * KP, 12, c->twiddle, c->stream);        /* along i */
+        launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, c->stream);
+        launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
     } else {
-        launch_colfft<double>(c->shat, c->N, KP, 12L * c->N, c->twiddle, c->stream);
-        launch_colfft<double>(c->shat, c->N, c->N * KP, 12, c->twiddle, c->stream);
+        launch_colfft<double>(vj, c->N, c->fft_cw, c->twiddle, c->stream);
+        launch_colfft<double>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
     }
     HIPCHK(hipGetLastError());
     return GABE_OK;
@@ -325,13 +340,16 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
         HIPCHK(hipMemset(c->vbuf[s], 0, fbytes));
     }
     const int narr = c->alloc_gw ? 12 : c->nf;
+    c->colfft = c->alloc_gw && gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
+    c->kpad = c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1;
+    if (const char *e = std::getenv("GABE_FFT_TILE")) c->fft_cw = std::atoi(e) == 32 ? 32 : 16;
     HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
-    HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * c->M));
+    HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad)));
     if (c->alloc_gw) {
-        HIPCHK(hipMalloc(&c->h, 2 * c->esz * 6 * c->M));
-        HIPCHK(hipMalloc(&c->l, 2 * c->esz * 6 * c->M));
-        HIPCHK(hipMemset(c->h, 0, 2 * c->esz * 6 * c->M));
-        HIPCHK(hipMemset(c->l, 0, 2 * c->esz * 6 * c->M));
+        HIPCHK(hipMalloc(&c->h, 2 * c->esz * 2 * c->M));
+        HIPCHK(hipMalloc(&c->l, 2 * c->esz * 2 * c->M));
+        HIPCHK(hipMemset(c->h, 0, 2 * c->esz * 2 * c->M));
+        HIPCHK(hipMemset(c->l, 0, 2 * c->esz * 2 * c->M));
     }
 
     /* sweep tiling: one thread per plane column, ci planes per workgroup */
@@ -361,9 +379,9 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
     HIPCHK(hipMalloc(&c->spec_out, sizeof(double) * nb * (GABE_MAXF + 2)));
 
     if (c->alloc_gw) {
-        c->colfft = gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
         if (c->colfft) {
-            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12 * (size_t)c->N2, 1));
+            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12 * (size_t)c->N2, 1,
+                          (size_t)c->kpad));
             std::vector<double> tw(2 * c->N);
             for (int e = 0; e < c->N; e++) {
                 tw[2 * e] = gabe_cos2pi_frac(e, c->N);
@@ -495,29 +513,24 @@ int gabe_upload_gw(gabe_ctx *c, const double *h, const double *l)
     if (!c->alloc_gw) return fail(GABE_ERR_STATE, "context created with calc_gws = 0");
     HIPCHK(hipStreamSynchronize(c->stream));
     const long M = c->M;
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * 12 * M));
     const double *src[2] = {h, l};
     void *dst[2] = {c->h, c->l};
-    for (int a = 0; a < 2; a++) {
+    int rc = GABE_OK;
+    for (int a = 0; a < 2 && rc == GABE_OK; a++) {
         if (!src[a]) continue;
-        if (c->fp32) {
-            std::vector<float> tmp(12 * M);
-            for (int cc = 0; cc < 6; cc++)
-                for (long m = 0; m < M; m++) {
-                    tmp[2 * (cc * M + m)] = (float)src[a][(2 * cc) * M + m];
-                    tmp[2 * (cc * M + m) + 1] = (float)src[a][(2 * cc + 1) * M + m];
-                }
-            HIPCHK(hipMemcpy(dst[a], tmp.data(), sizeof(float) * 12 * M, hipMemcpyHostToDevice));
-        } else {
-            std::vector<double> tmp(12 * M);
-            for (int cc = 0; cc < 6; cc++)
-                for (long m = 0; m < M; m++) {
-                    tmp[2 * (cc * M + m)] = src[a][(2 * cc) * M + m];
-                    tmp[2 * (cc * M + m) + 1] = src[a][(2 * cc + 1) * M + m];
-                }
-            HIPCHK(hipMemcpy(dst[a], tmp.data(), sizeof(double) * 12 * M, hipMemcpyHostToDevice));
+        if (hipMemcpy(tmp, src[a], sizeof(double) * 12 * M, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(GABE_ERR_HIP, "upload of tensor modes failed");
+            break;
         }
+        if (c->fp32) launch_gw_pack<float>(tmp, (float *)dst[a], c->N, M, c->stream);
+        else launch_gw_pack<double>(tmp, (double *)dst[a], c->N, M, c->stream);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(GABE_ERR_HIP, "tensor mode packing failed");
     }
-    return GABE_OK;
+    (void)hipFree(tmp);
+    return rc;
 }
 
 int gabe_download_gw(gabe_ctx *c, double *h, double *l)
@@ -526,29 +539,21 @@ int gabe_download_gw(gabe_ctx *c, double *h, double *l)
     if (!c->alloc_gw) return fail(GABE_ERR_STATE, "context created with calc_gws = 0");
     HIPCHK(hipStreamSynchronize(c->stream));
     const long M = c->M;
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * 12 * M));
     double *dst[2] = {h, l};
     const void *src[2] = {c->h, c->l};
-    for (int a = 0; a < 2; a++) {
+    int rc = GABE_OK;
+    for (int a = 0; a < 2 && rc == GABE_OK; a++) {
         if (!dst[a]) continue;
-        if (c->fp32) {
-            std::vector<float> tmp(12 * M);
-            HIPCHK(hipMemcpy(tmp.data(), src[a], sizeof(float) * 12 * M, hipMemcpyDeviceToHost));
-            for (int cc = 0; cc < 6; cc++)
-                for (long m = 0; m < M; m++) {
-                    dst[a][(2 * cc) * M + m] = tmp[2 * (cc * M + m)];
-                    dst[a][(2 * cc + 1) * M + m] = tmp[2 * (cc * M + m) + 1];
-                }
-        } else {
-            std::vector<double> tmp(12 * M);
-            HIPCHK(hipMemcpy(tmp.data(), src[a], sizeof(double) * 12 * M, hipMemcpyDeviceToHost));
-            for (int cc = 0; cc < 6; cc++)
-                for (long m = 0; m < M; m++) {
-                    dst[a][(2 * cc) * M + m] = tmp[2 * (cc * M + m)];
-                    dst[a][(2 * cc + 1) * M + m] = tmp[2 * (cc * M + m) + 1];
-                }
-        }
+        if (c->fp32) launch_gw_unpack<float>((const float *)src[a], tmp, c->N, M, c->stream);
+        else launch_gw_unpack<double>((const double *)src[a], tmp, c->N, M, c->stream);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+            hipMemcpy(dst[a], tmp, sizeof(double) * 12 * M, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(GABE_ERR_HIP, "tensor mode download failed");
     }
-    return GABE_OK;
+    (void)hipFree(tmp);
+    return rc;
 }
 
 int gabe_set_scalars(gabe_ctx *c, const gabe_scalars *s)
@@ -767,8 +772,8 @@ int gabe_tt_source(gabe_ctx *c, int s, double *out)
     if (rc == GABE_OK) rc = stress_fft(c);
     if (rc == GABE_OK) {
         const double dx3 = c->dx * c->dx * c->dx;
-        if (c->fp32) launch_tt_out<float>((const float *)c->shat, s, c->N, c->M, dx3, dout, c->stream);
-        else launch_tt_out<double>((const double *)c->shat, s, c->N, c->M, dx3, dout, c->stream);
+        if (c->fp32) launch_tt_out<float>((const float *)c->shat, s, c->N, c->M, c->kpad, dx3, dout, c->stream);
+        else launch_tt_out<double>((const double *)c->shat, s, c->N, c->M, c->kpad, dx3, dout, c->stream);
         if (hipGetLastError() != hipSuccess) rc = fail(GABE_ERR_HIP, "tt kernel launch failed");
     }
     if (rc == GABE_OK && hipMemcpyAsync(out, dout, sizeof(double) * 12 * c->M, hipMemcpyDeviceToHost, c->stream) != hipSuccess)

@@ -58,10 +58,20 @@ constexpr double gabe_sin2pi_frac(long e, long R)
 /* N for which the column transform exists (powers of two, 8..512) */
 bool gabe_colfft_supported(int N);
 
-/* In-place forward complex transform of length N along the middle axis of
- * data viewed as [outer][N][inner] (complex T).  tw is the device table
- * exp(-2 pi i e / N), e < N, in complex T. */
+/* Column set for one transform launch: element (o, r, s, c) of the view
+ * [outer][N][nsub][cols] at o*outer_stride + r*axis_stride + s*sub_stride + c
+ * (complex elements), transformed along r for every o, s and c < sub_valid. */
+struct ColFftView {
+    void *data;
+    long axis_stride, outer_stride, sub_stride;
+    int sub_valid, nsub;
+    long outer;
+};
+
+/* In-place forward complex transform of length N along r.  cw is the column
+ * tile width (16 or 32 complex).  tw is the device table exp(-2 pi i e / N),
+ * e < N, in complex T. */
 template <typename T>
-void launch_colfft(void *data, int N, long inner, long outer, const void *tw, hipStream_t s);
+void launch_colfft(const ColFftView &v, int N, int cw, const void *tw, hipStream_t s);
 
 #endif

@@ -99,22 +99,41 @@ __device__ __forceinline__ void dft_reg(C (&x)[R])
 
 constexpr int kColThreads = 256;
 
-/* data viewed as [outer][N][inner]; each workgroup transforms CW adjacent
- * inner columns of one outer slab along the middle axis, in place */
+/* Element (o, r, s, c) of the view [outer][N][sub][cols] sits at
+ * o*outer_stride + r*axis_stride + s*sub_stride + c.  Each workgroup takes CW
+ * adjacent columns (c < sub_valid) of one (o, s) and transforms them along r,
+ * in place.  Loads are issued in batches into registers before the LDS
+ * stores so that many global reads are in flight per wave. */
 template <typename T, int N1, int N2, int CW>
-__global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type *data, long inner, long outer_stride,
-                                                       const typename cplx_t<T>::type *tw)
+__global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type *data, long axis_stride,
+                                                       long outer_stride, long sub_stride, int sub_valid,
+                                                       int chunks_per_sub, const typename cplx_t<T>::type *tw)
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2;
+    constexpr int PER = (N * CW + kColThreads - 1) / kColThreads;   /* loads per thread */
+    constexpr int BATCH = PER < 16 ? PER : 16;
+    static_assert(PER % BATCH == 0, "tile must split into whole batches");
+    static_assert(sizeof(C) * N * (CW + 1) <= 160 * 1024, "tile exceeds the LDS of one CU");
     __shared__ C tile[N][CW + 1];
-    const long col0 = (long)blockIdx.x * CW;
-    const int width = (int)min((long)CW, inner - col0);
-    C *base = data + (long)blockIdx.y * outer_stride + col0;
+    const int sidx = blockIdx.x / chunks_per_sub, q = blockIdx.x % chunks_per_sub;
+    const int width = min(CW, sub_valid - q * CW);
+    C *base = data + (long)blockIdx.y * outer_stride + (long)sidx * sub_stride + (long)q * CW;
 
-    for (int e = threadIdx.x; e < N * CW; e += kColThreads) {
-        const int r = e / CW, cc = e % CW;
-        if (cc < width) tile[r][cc] = base[(long)r * inner + cc];
+#pragma unroll
+    for (int b0 = 0; b0 < PER; b0 += BATCH) {
+        C v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) {
+            const int e = threadIdx.x + (b0 + u) * kColThreads, r = e / CW, cc = e % CW;
+            if (e < N * CW && cc < width) v[u] = base[(long)r * axis_stride + cc];
+            else v[u].x = v[u].y = (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) {
+            const int e = threadIdx.x + (b0 + u) * kColThreads, r = e / CW, cc = e % CW;
+            if (e < N * CW) tile[r][cc] = v[u];
+        }
     }
     __syncthreads();
 
@@ -141,19 +160,30 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
         dft_reg<N2, T>(y);
         if (cc < width) {
 #pragma unroll
-            for (int k2 = 0; k2 < N2; k2++) base[(long)(k1 + N1 * k2) * inner + cc] = y[k2];
+            for (int k2 = 0; k2 < N2; k2++) base[(long)(k1 + N1 * k2) * axis_stride + cc] = y[k2];
         }
     }
 }
 
-template <typename T, int N1, int N2>
-void launch_n(void *data, long inner, long outer, const void *tw, hipStream_t s)
+template <typename T, int N1, int N2, int CW>
+void launch_cw(const ColFftView &v, const void *tw, hipStream_t s)
 {
     using C = typename cplx_t<T>::type;
-    constexpr int CW = 16;
-    const dim3 grid((unsigned)((inner + CW - 1) / CW), (unsigned)outer);
-    hipLaunchKernelGGL((k_colfft<T, N1, N2, CW>), grid, dim3(kColThreads), 0, s, (C *)data, inner,
-                       (long)(N1 * N2) * inner, (const C *)tw);
+    const int chunks = (v.sub_valid + CW - 1) / CW;
+    const dim3 grid((unsigned)(v.nsub * chunks), (unsigned)v.outer);
+    hipLaunchKernelGGL((k_colfft<T, N1, N2, CW>), grid, dim3(kColThreads), 0, s, (C *)v.data, v.axis_stride,
+                       v.outer_stride, v.sub_stride, v.sub_valid, chunks, (const C *)tw);
+}
+
+template <typename T, int N1, int N2>
+void launch_n(const ColFftView &v, int cw, const void *tw, hipStream_t s)
+{
+    constexpr size_t csz = 2 * sizeof(T);
+    if constexpr (csz * N1 * N2 * 33 <= 160 * 1024) {
+        if (cw == 32) { launch_cw<T, N1, N2, 32>(v, tw, s); return; }
+    }
+    if constexpr (csz * N1 * N2 * 17 <= 160 * 1024) launch_cw<T, N1, N2, 16>(v, tw, s);
+    else launch_cw<T, N1, N2, 8>(v, tw, s);
 }
 
 } // namespace
@@ -164,19 +194,19 @@ bool gabe_colfft_supported(int N)
 }
 
 template <typename T>
-void launch_colfft(void *data, int N, long inner, long outer, const void *tw, hipStream_t s)
+void launch_colfft(const ColFftView &v, int N, int cw, const void *tw, hipStream_t s)
 {
     switch (N) {
-    case 8: launch_n<T, 4, 2>(data, inner, outer, tw, s); break;
-    case 16: launch_n<T, 4, 4>(data, inner, outer, tw, s); break;
-    case 32: launch_n<T, 8, 4>(data, inner, outer, tw, s); break;
-    case 64: launch_n<T, 8, 8>(data, inner, outer, tw, s); break;
-    case 128: launch_n<T, 16, 8>(data, inner, outer, tw, s); break;
-    case 256: launch_n<T, 16, 16>(data, inner, outer, tw, s); break;
-    case 512: launch_n<T, 32, 16>(data, inner, outer, tw, s); break;
+    case 8: launch_n<T, 4, 2>(v, cw, tw, s); break;
+    case 16: launch_n<T, 4, 4>(v, cw, tw, s); break;
+    case 32: launch_n<T, 8, 4>(v, cw, tw, s); break;
+    case 64: launch_n<T, 8, 8>(v, cw, tw, s); break;
+    case 128: launch_n<T, 16, 8>(v, cw, tw, s); break;
+    case 256: launch_n<T, 16, 16>(v, cw, tw, s); break;
+    case 512: launch_n<T, 32, 16>(v, cw, tw, s); break;
     default: break;
     }
 }
 
-template void launch_colfft<float>(void *, int, long, long, const void *, hipStream_t);
-template void launch_colfft<double>(void *, int, long, long, const void *, hipStream_t);
+template void launch_colfft<float>(const ColFftView &, int, int, const void *, hipStream_t);
+template void launch_colfft<double>(const ColFftView &, int, int, const void *, hipStream_t);

@@ -11,6 +11,7 @@
 
 #define GABE_MAXF 2
 #define GABE_BLOCK 256
+#define GABE_NXCD 8         /* accelerator complex dies of an MI355X, each with its own L2 */
 #define GABE_HIST_COLS 8
 
 /* Persistent device scalars: the reference's a[2], adot[2], ed*[2]
@@ -74,18 +75,19 @@ struct FinArgs {
 
 template <typename T>
 struct GwArgs {
-    const T *shat;      /* [12][M] complex: S(f0) transforms 0-5, S(f1) 6-11 */
-    T *h, *l;           /* [6][M] complex, components 11,12,13,22,23,33 */
+    const T *shat;      /* [12][N][N][kpad] complex: S(f0) transforms 0-5, S(f1) 6-11 */
+    T *h, *l;           /* [M][2] complex: (+, x) polarisation amplitudes per mode */
     const StepScalars *ss;
     int N;
-    long M;
+    long M;             /* N*N*(N/2+1) modes */
+    long kpad;          /* row stride of shat (complex elements) */
     double L, dt, dx3;
 };
 
 template <typename T>
 struct SpecArgs {
     const T *fhat;      /* [nf][M] complex field transforms */
-    const T *l;         /* [6][M] complex, or null */
+    const T *l;         /* [M][2] complex polarisation pairs, or null */
     double dc[GABE_MAXF];   /* added to the real part of each field's k=0 mode */
     int nf, N, numbins, quirk;
     long M;
@@ -98,7 +100,11 @@ void launch_finalize(const FinArgs &a, hipStream_t s);
 template <typename T>
 void launch_gw(const GwArgs<T> &a, hipStream_t s);
 template <typename T>
-void launch_tt_out(const T *shat, int stage, int N, long M, double dx3, double *out, hipStream_t s);
+void launch_gw_pack(const double *ref, T *pol, int N, long M, hipStream_t s);
+template <typename T>
+void launch_gw_unpack(const T *pol, double *ref, int N, long M, hipStream_t s);
+template <typename T>
+void launch_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3, double *out, hipStream_t s);
 template <typename T>
 void launch_spectra(const SpecArgs<T> &a, hipStream_t s);
 template <typename T>

@@ -73,7 +73,20 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
     constexpr int NS = gabe_nsums(NF);
     const int N = A.N;
     const long N2 = A.N2;
-    const long p = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    /* XCD-aware block order: consecutive hardware workgroups go round-robin to
+     * the 8 XCDs, each with its own L2.  Give every XCD a contiguous range of
+     * rows instead, so the j+-1 neighbour rows a workgroup reads are being
+     * read by workgroups on the same XCD at the same time. */
+    int bx = blockIdx.x, by = blockIdx.y;
+    {
+        const int nb = gridDim.x * gridDim.y, hw = blockIdx.y * gridDim.x + blockIdx.x;
+        if (nb % GABE_NXCD == 0) {
+            const int logical = (hw % GABE_NXCD) * (nb / GABE_NXCD) + hw / GABE_NXCD;
+            bx = logical % gridDim.x;
+            by = logical / gridDim.x;
+        }
+    }
+    const long p = (long)bx * GABE_BLOCK + threadIdx.x;
     const bool valid = p < N2;
     const long pp = valid ? p : 0;
     const int j = (int)(pp / N), k = (int)(pp - (long)j * N);
@@ -81,9 +94,9 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
     const long pkm = (long)j * N + (k == 0 ? N - 1 : k - 1);
     const long pjp = (long)(j + 1 == N ? 0 : j + 1) * N + k;
     const long pjm = (long)(j == 0 ? N - 1 : j - 1) * N + k;
-    const int i0 = blockIdx.y * A.ci;
+    const int i0 = by * A.ci;
     const int i1 = min(i0 + A.ci, N);
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int bid = by * gridDim.x + bx;
 
     const T idx2 = (T)A.inv_dx2;
     const double dt = A.dt, hdt = 0.5 * dt;
@@ -407,42 +420,85 @@ __device__ __forceinline__ void tt_project(const C S[6], R nx, R ny, R nz, R dx3
     }
 }
 
-/* One thread per mode: TT source of both stages, then the two midpoint-RK2
- * stages of evolve_perts (g2evolutionpert.cpp:343-455, derivs :459-516) in
- * registers; reads S0, S1, h, l once and writes h, l once. */
+/* TT polarisation basis of mode n = (px,py,pz) != 0: u, v orthonormal and
+ * orthogonal to n (u from the coordinate axis least aligned with n, v = n x u),
+ * e+ = u u - v v, ex = u v + v u, stored as components 11,12,13,22,23,33.
+ * With these, Lambda_ij,lm = (e+_ij e+_lm + ex_ij ex_lm) / 2: the TT part of
+ * a symmetric S is sum_pol e_pol (e_pol : S) / 2. */
+__device__ __forceinline__ void pol_basis(int px, int py, int pz, double ep[6], double ex[6])
+{
+    const double inv = 1. / sqrt((double)(px * px + py * py + pz * pz));
+    const double n[3] = {px * inv, py * inv, pz * inv};
+    int a = 0;
+    if (fabs(n[1]) < fabs(n[a])) a = 1;
+    if (fabs(n[2]) < fabs(n[a])) a = 2;
+    double u[3] = {-n[a] * n[0], -n[a] * n[1], -n[a] * n[2]};
+    u[a] += 1.;
+    const double un = 1. / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    u[0] *= un; u[1] *= un; u[2] *= un;
+    const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+    const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        ep[c] = u[ia[c]] * u[ib[c]] - v[ia[c]] * v[ib[c]];
+        ex[c] = u[ia[c]] * v[ib[c]] + v[ia[c]] * u[ib[c]];
+    }
+}
+
+__device__ __forceinline__ void mode_of(long m, int N, int &px, int &py, int &pz, long &row)
+{
+    const int H = N / 2 + 1;
+    pz = (int)(m % H);
+    row = m / H;
+    const int j = (int)(row % N), i = (int)(row / N);
+    px = i <= N / 2 ? i : i - N;
+    py = j <= N / 2 ? j : j - N;
+}
+
+/* One thread per mode: the TT source of both stages projected on the two
+ * polarisations, then the two midpoint-RK2 stages of evolve_perts
+ * (g2evolutionpert.cpp:343-455, derivs :459-516) in registers.  h and l are
+ * stored per mode as (h+, hx) complex pairs: a TT tensor has two degrees of
+ * freedom, and with a TT source and TT (e.g. zero) start every component of
+ * the reference's h_ij stays in that span. */
 template <typename T>
 __global__ __launch_bounds__(GABE_BLOCK) void k_gw(GwArgs<T> G)
 {
     using C = typename cplx_of<T>::type;
     const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
     if (m >= G.M) return;
-    const int N = G.N, H = N / 2 + 1;
-    const int kz = (int)(m % H);
-    const long r = m / H;
-    const int j = (int)(r % N), i = (int)(r / N);
-    const int px = i <= N / 2 ? i : i - N, py = j <= N / 2 ? j : j - N, pz = kz;
+    const int N = G.N;
+    int px, py, pz;
+    long r;
+    mode_of(m, N, px, py, pz, r);
     const int p2 = px * px + py * py + pz * pz;
 
     const C *Sh = reinterpret_cast<const C *>(G.shat);
     C *hh = reinterpret_cast<C *>(G.h);
     C *ll = reinterpret_cast<C *>(G.l);
-    C S0[6], S1[6], h0[6], l0[6];
+    const long SM = (long)N * N * G.kpad, ms = r * G.kpad + pz;
+    C S0[6], S1[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-        S0[c] = Sh[(long)c * G.M + m];
-        S1[c] = Sh[(long)(6 + c) * G.M + m];
-        h0[c] = hh[(long)c * G.M + m];
-        l0[c] = ll[(long)c * G.M + m];
+        S0[c] = Sh[(long)c * SM + ms];
+        S1[c] = Sh[(long)(6 + c) * SM + ms];
     }
-    C T0[6], T1[6];
-    if (p2 == 0) {   /* zero mode (g2evolutionpert.cpp:319-321) */
+    C h0[2] = {hh[2 * m], hh[2 * m + 1]}, l0[2] = {ll[2 * m], ll[2 * m + 1]};
+    C T0[2], T1[2];
 #pragma unroll
-        for (int c = 0; c < 6; c++) { T0[c].x = T0[c].y = 0; T1[c].x = T1[c].y = 0; }
-    } else {
-        const double inv = 1. / sqrt((double)p2);
-        const T nx = (T)(px * inv), ny = (T)(py * inv), nz = (T)(pz * inv);
-        tt_project<C, T>(S0, nx, ny, nz, (T)G.dx3, T0);
-        tt_project<C, T>(S1, nx, ny, nz, (T)G.dx3, T1);
+    for (int q = 0; q < 2; q++) { T0[q].x = T0[q].y = 0; T1[q].x = T1[q].y = 0; }
+    if (p2 != 0) {   /* the zero mode has no source (g2evolutionpert.cpp:319-321) */
+        double e[2][6];
+        pol_basis(px, py, pz, e[0], e[1]);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                /* (e : S)/2 dx^3 with off-diagonal components counted twice */
+                const T w = (T)((c == 0 || c == 3 || c == 5 ? 0.5 : 1.0) * G.dx3 * e[q][c]);
+                T0[q].x += w * S0[c].x; T0[q].y += w * S0[c].y;
+                T1[q].x += w * S1[c].x; T1[q].y += w * S1[c].y;
+            }
     }
     const StepScalars *ss = G.ss;
     const double k2n = 4. * M_PI * M_PI / G.L / G.L * (double)p2;
@@ -451,24 +507,74 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw(GwArgs<T> G)
     const T om0 = (T)(16. * M_PI / (ss->a0 * ss->a0)), om1 = (T)(16. * M_PI / (ss->a1 * ss->a1));
     const T hdt = (T)(0.5 * G.dt), dt = (T)G.dt;
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
+    for (int q = 0; q < 2; q++) {
         C h1, l1, hn, ln;
-        h1.x = h0[c].x + hdt * l0[c].x;
-        h1.y = h0[c].y + hdt * l0[c].y;
-        l1.x = l0[c].x + hdt * (-kk0 * h0[c].x - fr0 * l0[c].x + om0 * T0[c].x);
-        l1.y = l0[c].y + hdt * (-kk0 * h0[c].y - fr0 * l0[c].y + om0 * T0[c].y);
-        hn.x = h0[c].x + dt * l1.x;
-        hn.y = h0[c].y + dt * l1.y;
-        ln.x = l0[c].x + dt * (-kk1 * h1.x - fr1 * l1.x + om1 * T1[c].x);
-        ln.y = l0[c].y + dt * (-kk1 * h1.y - fr1 * l1.y + om1 * T1[c].y);
-        hh[(long)c * G.M + m] = hn;
-        ll[(long)c * G.M + m] = ln;
+        h1.x = h0[q].x + hdt * l0[q].x;
+        h1.y = h0[q].y + hdt * l0[q].y;
+        l1.x = l0[q].x + hdt * (-kk0 * h0[q].x - fr0 * l0[q].x + om0 * T0[q].x);
+        l1.y = l0[q].y + hdt * (-kk0 * h0[q].y - fr0 * l0[q].y + om0 * T0[q].y);
+        hn.x = h0[q].x + dt * l1.x;
+        hn.y = h0[q].y + dt * l1.y;
+        ln.x = l0[q].x + dt * (-kk1 * h1.x - fr1 * l1.x + om1 * T1[q].x);
+        ln.y = l0[q].y + dt * (-kk1 * h1.y - fr1 * l1.y + om1 * T1[q].y);
+        hh[2 * m + q] = hn;
+        ll[2 * m + q] = ln;
+    }
+}
+
+/* reference planes [12][M] (11,12,13,22,23,33 x Re,Im) <-> polarisation pairs
+ * [M][2] complex: h_pol = (e_pol : h)/2, h_ij = sum_pol h_pol e_pol_ij */
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_gw_pack(const double *ref, T *pol, int N, long M)
+{
+    using C = typename cplx_of<T>::type;
+    const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    if (m >= M) return;
+    int px, py, pz;
+    long r;
+    mode_of(m, N, px, py, pz, r);
+    C out[2];
+    out[0].x = out[0].y = out[1].x = out[1].y = 0;
+    if (px != 0 || py != 0 || pz != 0) {
+        double e[2][6];
+        pol_basis(px, py, pz, e[0], e[1]);
+        for (int q = 0; q < 2; q++) {
+            double re = 0., im = 0.;
+            for (int c = 0; c < 6; c++) {
+                const double w = (c == 0 || c == 3 || c == 5 ? 0.5 : 1.0) * e[q][c];
+                re += w * ref[(long)(2 * c) * M + m];
+                im += w * ref[(long)(2 * c + 1) * M + m];
+            }
+            out[q].x = (T)re;
+            out[q].y = (T)im;
+        }
+    }
+    reinterpret_cast<C *>(pol)[2 * m] = out[0];
+    reinterpret_cast<C *>(pol)[2 * m + 1] = out[1];
+}
+
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_gw_unpack(const T *pol, double *ref, int N, long M)
+{
+    using C = typename cplx_of<T>::type;
+    const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
+    if (m >= M) return;
+    int px, py, pz;
+    long r;
+    mode_of(m, N, px, py, pz, r);
+    double e[2][6] = {{0., 0., 0., 0., 0., 0.}, {0., 0., 0., 0., 0., 0.}};
+    if (px != 0 || py != 0 || pz != 0) pol_basis(px, py, pz, e[0], e[1]);
+    const C a = reinterpret_cast<const C *>(pol)[2 * m], b = reinterpret_cast<const C *>(pol)[2 * m + 1];
+    for (int c = 0; c < 6; c++) {
+        ref[(long)(2 * c) * M + m] = e[0][c] * (double)a.x + e[1][c] * (double)b.x;
+        ref[(long)(2 * c + 1) * M + m] = e[0][c] * (double)a.y + e[1][c] * (double)b.y;
     }
 }
 
 /* T_gw[s] in the reference layout [12][M] (diagnostic, not on the step path) */
 template <typename T>
-__global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, int stage, int N, long M, double dx3, double *out)
+__global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3,
+                                                         double *out)
 {
     using C = typename cplx_of<T>::type;
     const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
@@ -479,10 +585,11 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, int stage,
     const int j = (int)(r % N), i = (int)(r / N);
     const int px = i <= N / 2 ? i : i - N, py = j <= N / 2 ? j : j - N, pz = kz;
     const int p2 = px * px + py * py + pz * pz;
-    const C *Sh = reinterpret_cast<const C *>(shat) + (long)(6 * stage) * M;
+    const long SM = (long)N * N * kpad;
+    const C *Sh = reinterpret_cast<const C *>(shat) + (long)(6 * stage) * SM;
     C S[6], Tt[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) S[c] = Sh[(long)c * M + m];
+    for (int c = 0; c < 6; c++) S[c] = Sh[(long)c * SM + r * kpad + kz];
     if (p2 == 0) {
 #pragma unroll
         for (int c = 0; c < 6; c++) Tt[c].x = Tt[c].y = 0;
@@ -548,14 +655,12 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_spectra(SpecArgs<T> P)
             }
             acc[nf] += w;
             if (Lg) {
+                /* sum_ij |l_ij|^2 (off-diagonal components twice, g2spectra.cpp:133-144)
+                 * = 2 (|l+|^2 + |lx|^2) for the polarisation pair */
                 const long ms = (P.quirk && kz == 0) ? row + N / 2 : m;
-                double e = 0.;
-#pragma unroll
-                for (int c = 0; c < 6; c++) {
-                    const C v = Lg[(long)c * M + ms];
-                    const double cw = (c == 0 || c == 3 || c == 5) ? 1. : 2.;
-                    e += cw * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
-                }
+                const C a = Lg[2 * ms], bb = Lg[2 * ms + 1];
+                const double e = 2. * ((double)a.x * (double)a.x + (double)a.y * (double)a.y +
+                                       (double)bb.x * (double)bb.x + (double)bb.y * (double)bb.y);
                 acc[nf + 1] += w * e;
             }
         }
@@ -627,10 +732,24 @@ void launch_gw(const GwArgs<T> &a, hipStream_t s)
 }
 
 template <typename T>
-void launch_tt_out(const T *shat, int stage, int N, long M, double dx3, double *out, hipStream_t s)
+void launch_gw_pack(const double *ref, T *pol, int N, long M, hipStream_t s)
 {
     const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
-    hipLaunchKernelGGL((k_tt_out<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, shat, stage, N, M, dx3, out);
+    hipLaunchKernelGGL((k_gw_pack<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, ref, pol, N, M);
+}
+
+template <typename T>
+void launch_gw_unpack(const T *pol, double *ref, int N, long M, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
+    hipLaunchKernelGGL((k_gw_unpack<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, pol, ref, N, M);
+}
+
+template <typename T>
+void launch_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3, double *out, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
+    hipLaunchKernelGGL((k_tt_out<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, shat, stage, N, M, kpad, dx3, out);
 }
 
 template <typename T>
@@ -657,7 +776,9 @@ void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
 #define GABE_INST(T)                                                                                          \
     template void launch_sweep<T>(int, int, bool, const SweepArgs<T> &, dim3, hipStream_t);                   \
     template void launch_gw<T>(const GwArgs<T> &, hipStream_t);                                               \
-    template void launch_tt_out<T>(const T *, int, int, long, double, double *, hipStream_t);                 \
+    template void launch_gw_pack<T>(const double *, T *, int, long, hipStream_t);                            \
+    template void launch_gw_unpack<T>(const T *, double *, int, long, hipStream_t);                          \
+    template void launch_tt_out<T>(const T *, int, int, long, long, double, double *, hipStream_t);           \
     template void launch_spectra<T>(const SpecArgs<T> &, hipStream_t);                                        \
     template void launch_effmass<T>(const T *, const T *, int, int, double, double, gabe_model, long,        \
                                     double *, int, hipStream_t);                                              \

@@ -90,7 +90,11 @@ int gabe_upload_state(gabe_ctx *ctx, const double *field, const double *dfield);
 int gabe_download_state(gabe_ctx *ctx, double *field, double *dfield);
 
 /* Slot-0 tensor perturbations h and l = dh/dt, layout [12][N][N][N/2+1]
- * (g2main.cpp:30-35).  Either pointer may be NULL. */
+ * (g2main.cpp:30-35).  Either pointer may be NULL.  The device keeps each mode
+ * as its two transverse-traceless polarisation amplitudes: the reference's
+ * zero start and TT source (g2evolutionpert.cpp:215-321) never leave that
+ * span, so an upload stores the TT part of h and l (k = 0 mode zero) and a
+ * download rebuilds the six components from it. */
 int gabe_upload_gw(gabe_ctx *ctx, const double *h, const double *l);
 int gabe_download_gw(gabe_ctx *ctx, double *h, double *l);
 

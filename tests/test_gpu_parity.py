@@ -197,3 +197,27 @@ def test_history_and_deterministic_replay():
     assert ha.shape == (10, 8)
     assert np.array_equal(ha, hb)
     assert np.all(np.diff(ha[:, 7]) == 1)
+
+
+def test_gw_upload_keeps_the_tt_part():
+    """Upload/download of tensor modes: a TT tensor round-trips, any other
+    tensor comes back as its TT projection (the oracle's source is TT)."""
+    cfg = configs.lambda_phi4_config(N=16, calc_gws=1)
+    o = seeded_oracle(cfg, amp=2e-3)
+    o.fft_stresstensor(0)
+    tt = o.T_gw[0].copy()
+    lat = lattice_from_oracle(o, cfg, GABE_FP64)
+    lat.upload_gw(tt, 2.0 * tt)
+    h, l = lat.download_gw()
+    assert rel_err(h, tt) < 1e-13 and rel_err(l, 2.0 * tt) < 1e-13
+    rng = np.random.default_rng(7)
+    junk = rng.standard_normal(tt.shape)
+    lat.upload_gw(tt + 0.0 * junk, None)
+    # a pure-trace tensor (delta_ij f(k)) has no TT part
+    trace = np.zeros_like(tt)
+    for c in (0, 3, 5):
+        trace[2 * c] = junk[0]
+        trace[2 * c + 1] = junk[1]
+    lat.upload_gw(trace, None)
+    h2, _ = lat.download_gw()
+    assert np.abs(h2).max() < 1e-13 * np.abs(junk).max()
