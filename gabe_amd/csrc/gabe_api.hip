@@ -15,6 +15,7 @@
 
 #include "gabe.h"
 #include "gabe_fft.h"
+#include "gabe_gwfft.h"
 #include "gabe_internal.h"
 
 namespace {
@@ -100,6 +101,7 @@ struct gabe_ctx {
     double *spec_out = nullptr;
     FftPlan fft_stress, fft_spec, fft_c2r;
     bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
+    bool gwfft = false;          /* i columns fused with the tensor-mode update */
     long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
     int fft_cw = 16;             /* column tile width of the j/i transforms */
     void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */
@@ -233,8 +235,9 @@ int launch_gw_ctx(gabe_ctx *c)
     return GABE_OK;
 }
 
-/* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid) */
-int stress_fft(gabe_ctx *c)
+/* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid).
+ * with_i = false stops after the j columns (the fused kernel does i). */
+int stress_fft(gabe_ctx *c, bool with_i = true)
 {
     if (!c->colfft) return run_plan(c, c->fft_stress, c->stress, c->shat);
     CHK(run_plan(c, c->fft_stress, c->stress, c->shat));            /* rows along k */
@@ -243,10 +246,28 @@ int stress_fft(gabe_ctx *c)
     ColFftView vi{c->shat, N * kp, N * N * kp, kp, (int)KP, (int)N, 12}; /* along i */
     if (c->fp32) {
         launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, c->stream);
-        launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
+        if (with_i) launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
     } else {
         launch_colfft<double>(vj, c->N, c->fft_cw, c->twiddle, c->stream);
-        launch_colfft<double>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
+        if (with_i) launch_colfft<double>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+/* tensor modes of one step: i transform + TT projection + both RK2 stages */
+int gw_modes(gabe_ctx *c)
+{
+    if (!c->gwfft) return launch_gw_ctx(c);
+    const double dx3 = c->dx * c->dx * c->dx;
+    if (c->fp32) {
+        GwFftArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, (const float *)c->twiddle,
+                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3};
+        launch_gw_ifft<float>(g, c->stream);
+    } else {
+        GwFftArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss,
+                            (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3};
+        launch_gw_ifft<double>(g, c->stream);
     }
     HIPCHK(hipGetLastError());
     return GABE_OK;
@@ -266,10 +287,10 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     CHK(launch_finalize_ctx(c, SW_A, 0));
     CHK(rec(3));
     CHK(rec(4));
-    if (c->gw) CHK(stress_fft(c));
+    if (c->gw) CHK(stress_fft(c, !c->gwfft));
     CHK(rec(5));
     CHK(rec(6));
-    if (c->gw) CHK(launch_gw_ctx(c));
+    if (c->gw) CHK(gw_modes(c));
     CHK(rec(7));
     CHK(rec(8));
     CHK(launch_sweep_ctx(c, SW_B, false));
@@ -342,6 +363,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
     const int narr = c->alloc_gw ? 12 : c->nf;
     c->colfft = c->alloc_gw && gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
     c->kpad = c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1;
+    c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
     if (const char *e = std::getenv("GABE_FFT_TILE")) c->fft_cw = std::atoi(e) == 32 ? 32 : 16;
     HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
     HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad)));

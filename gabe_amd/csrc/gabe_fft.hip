@@ -17,85 +17,11 @@
 #include <hip/hip_runtime.h>
 
 #include "gabe_fft.h"
+#include "gabe_fft_dev.h"
 
 namespace {
 
-template <typename T>
-struct cplx_t;
-template <>
-struct cplx_t<float> { using type = float2; };
-template <>
-struct cplx_t<double> { using type = double2; };
-
-/* table of exp(-2 pi i e / R), e < R, for the in-register DFTs */
-template <int R>
-struct Twiddles {
-    double re[R], im[R];
-    constexpr Twiddles() : re(), im()
-    {
-        for (int e = 0; e < R; e++) {
-            re[e] = gabe_cos2pi_frac(e, R);
-            im[e] = -gabe_sin2pi_frac(e, R);
-        }
-    }
-};
-
-template <int R>
-constexpr int bitrev(int i)
-{
-    int r = 0;
-    for (int b = 1; b < R; b <<= 1) {
-        r = (r << 1) | (i & 1);
-        i >>= 1;
-    }
-    return r;
-}
-
-template <typename C>
-__device__ __forceinline__ C cmul(C a, C b)
-{
-    C r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
-    return r;
-}
-
-/* in-place forward DFT of R (power of two) values held in registers */
-template <int R, typename T, typename C>
-__device__ __forceinline__ void dft_reg(C (&x)[R])
-{
-    constexpr Twiddles<R> tw{};
-#pragma unroll
-    for (int i = 0; i < R; i++) {
-        const int j = bitrev<R>(i);
-        if (j > i) {
-            const C t = x[i];
-            x[i] = x[j];
-            x[j] = t;
-        }
-    }
-#pragma unroll
-    for (int len = 2; len <= R; len <<= 1) {
-#pragma unroll
-        for (int i = 0; i < R; i += len) {
-#pragma unroll
-            for (int k = 0; k < len / 2; k++) {
-                const C u = x[i + k];
-                C v = x[i + k + len / 2];
-                if (k != 0) {
-                    C w;
-                    w.x = (T)tw.re[k * (R / len)];
-                    w.y = (T)tw.im[k * (R / len)];
-                    v = cmul(v, w);
-                }
-                x[i + k].x = u.x + v.x;
-                x[i + k].y = u.y + v.y;
-                x[i + k + len / 2].x = u.x - v.x;
-                x[i + k + len / 2].y = u.y - v.y;
-            }
-        }
-    }
-}
+using namespace gabe_fft;
 
 constexpr int kColThreads = 256;
 
@@ -116,7 +42,10 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
     static_assert(PER % BATCH == 0, "tile must split into whole batches");
     static_assert(sizeof(C) * N * (CW + 1) <= 160 * 1024, "tile exceeds the LDS of one CU");
     __shared__ C tile[N][CW + 1];
-    const int sidx = blockIdx.x / chunks_per_sub, q = blockIdx.x % chunks_per_sub;
+    /* XCD-aware order: neighbouring column chunks on one XCD (one L2) */
+    int b = blockIdx.x;
+    if (gridDim.x % 8 == 0) b = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+    const int sidx = b / chunks_per_sub, q = b % chunks_per_sub;
     const int width = min(CW, sub_valid - q * CW);
     C *base = data + (long)blockIdx.y * outer_stride + (long)sidx * sub_stride + (long)q * CW;
 

@@ -1,0 +1,31 @@
+/*
+ * gabe_gwfft.h -- the i-direction stress transform fused with the tensor-mode
+ * update (gabe_gwfft.hip).
+ */
+#ifndef GABE_GWFFT_H
+#define GABE_GWFFT_H
+
+#include <hip/hip_runtime.h>
+
+#include "gabe_internal.h"
+
+template <typename T>
+struct GwFftArgs {
+    const T *shat;      /* [12][N][N][kpad] complex, transformed along k and j */
+    T *h, *l;           /* [M][2] complex polarisation pairs */
+    const StepScalars *ss;
+    const T *tw;        /* exp(-2 pi i e / N), e < N, complex */
+    int N;
+    long M, kpad;
+    double L, dt, dx3;
+};
+
+/* whether the fused kernel exists for this size and storage type: FP32,
+ * N a power of two in 16..256 (the FP64 form uses the separate i transform
+ * and the mode kernel) */
+bool gabe_gwfft_supported(int N, bool fp32);
+
+template <typename T>
+void launch_gw_ifft(const GwFftArgs<T> &a, hipStream_t s);
+
+#endif

@@ -1,0 +1,248 @@
+/*
+ * gabe_gwfft.hip -- last stress transform (along i) fused with the tensor-mode
+ * update.
+ *
+ * After the k-row R2C and the j-column transform, one workgroup takes the
+ * N x CW tile of modes (all i, CW adjacent kz) at one j.  It transforms the 12
+ * stress components along i in LDS, CPR components per round.  After each
+ * round, every thread projects the freshly transformed components of its own
+ * modes onto the two TT polarisations and accumulates them.  Once all 12 are
+ * in, it runs both midpoint-RK2 stages of those modes
+ * (g2evolutionpert.cpp:343-516) and writes h and l.  The fully transformed
+ * source never goes back to HBM.
+ */
+#include <hip/hip_runtime.h>
+
+#include "gabe_fft_dev.h"
+#include "gabe_gwfft.h"
+#include "gabe_modes.h"
+
+namespace {
+
+using namespace gabe_fft;
+
+/* largest divisor of n not above 8: tile loads in flight per batch */
+constexpr int load_batch(int n)
+{
+    for (int b = 8; b > 1; b--)
+        if (n % b == 0) return b;
+    return 1;
+}
+
+/* One RK2 stage: components 6 ST .. 6 ST + 5 of the source: load the six
+ * N x CW tiles, transform along i (two steps, twiddles read from the first
+ * tile's padding column), return to natural order, and project the owned
+ * modes on e+ and ex into acc[mode][pol]. */
+template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST>
+__device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CPR][N1 * N2][CW + 1],
+                                             const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
+                                             const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
+                                             const T (&bv)[(N1 * N2 * CW + NT - 1) / NT][3],
+                                             typename cplx_t<T>::type (&acc)[(N1 * N2 * CW + NT - 1) / NT][2])
+{
+    using C = typename cplx_t<T>::type;
+    constexpr int N = N1 * N2;
+    constexpr int MPT = (N * CW + NT - 1) / NT;
+    constexpr int LPT = (CPR * N * CW + NT - 1) / NT;
+    constexpr int LB = load_batch(LPT);
+    constexpr int S1T = (CPR * N2 * CW + NT - 1) / NT;
+    constexpr int S2T = (CPR * N1 * CW + NT - 1) / NT;
+    static_assert(CPR == 6, "one round transforms the six components of a stage");
+    static_assert(LPT % LB == 0, "loads must split into whole batches");
+    {
+        constexpr int c0 = 0;
+#pragma unroll
+        for (int t0 = 0; t0 < LPT; t0 += LB) {
+            C v[LB];
+#pragma unroll
+            for (int t = 0; t < LB; t++) {
+                const int e = threadIdx.x + (t0 + t) * NT;
+                const int r = e / (N * CW), rem = e % (N * CW), i = rem / CW, col = rem % CW;
+                if (e < CPR * N * CW && col < width)
+                    v[t] = Sh[(long)(6 * ST + c0 + r) * SM + (long)i * rstride + col];
+                else v[t].x = v[t].y = (T)0;
+            }
+#pragma unroll
+            for (int t = 0; t < LB; t++) {
+                const int e = threadIdx.x + (t0 + t) * NT;
+                const int r = e / (N * CW), rem = e % (N * CW);
+                if (e < CPR * N * CW) tile[r][rem / CW][rem % CW] = v[t];
+            }
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int t = 0; t < S1T; t++) {
+            const int task = threadIdx.x + t * NT;
+            if (task < CPR * N2 * CW) {
+                const int r = task / (N2 * CW), rem = task % (N2 * CW), col = rem % CW, n2 = rem / CW;
+                C x[N1];
+#pragma unroll
+                for (int n1 = 0; n1 < N1; n1++) x[n1] = tile[r][N2 * n1 + n2][col];
+                dft_reg<N1, T>(x);
+#pragma unroll
+                for (int k1 = 0; k1 < N1; k1++) {
+                    C w = x[k1];
+                    if (k1 != 0 && n2 != 0) w = cmul(w, tile[0][(n2 * k1) % N][CW]);
+                    tile[r][N2 * k1 + n2][col] = w;
+                }
+            }
+        }
+        __syncthreads();
+
+        C y[S2T][N2];
+#pragma unroll
+        for (int t = 0; t < S2T; t++) {
+            const int task = threadIdx.x + t * NT;
+            if (task < CPR * N1 * CW) {
+                const int r = task / (N1 * CW), rem = task % (N1 * CW), col = rem % CW, k1 = rem / CW;
+#pragma unroll
+                for (int n2 = 0; n2 < N2; n2++) y[t][n2] = tile[r][N2 * k1 + n2][col];
+                dft_reg<N2, T>(y[t]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < S2T; t++) {
+            const int task = threadIdx.x + t * NT;
+            if (task < CPR * N1 * CW) {
+                const int r = task / (N1 * CW), rem = task % (N1 * CW), col = rem % CW, k1 = rem / CW;
+#pragma unroll
+                for (int k2 = 0; k2 < N2; k2++) tile[r][k1 + N1 * k2][col] = y[t][k2];
+            }
+        }
+        __syncthreads();
+
+        /* (e : S)/2 of the owned modes, off-diagonal components counted twice */
+#pragma unroll
+        for (int u = 0; u < MPT; u++) {
+            const int e = threadIdx.x + u * NT;
+            if (e < N * CW) {
+                const int i = e / CW, col = e % CW;
+#pragma unroll
+                for (int r = 0; r < CPR; r++) {
+                    const int cc = c0 + r;   /* 11,12,13,22,23,33 */
+                    const int ia = cc < 3 ? 0 : (cc < 5 ? 1 : 2);
+                    const int ib = cc == 0 ? 0 : (cc == 1 || cc == 3 ? 1 : 2);
+                    const T ua = ia == 0 ? bu[u][0] : (ia == 1 ? bu[u][1] : bu[u][2]);
+                    const T va = ia == 0 ? bv[u][0] : (ia == 1 ? bv[u][1] : bv[u][2]);
+                    const T ub = ib == 0 ? bu[u][0] : (ib == 1 ? bu[u][1] : bu[u][2]);
+                    const T vb = ib == 0 ? bv[u][0] : (ib == 1 ? bv[u][1] : bv[u][2]);
+                    const T w = ia == ib ? (T)0.5 : (T)1;
+                    const T ep = w * (ua * ub - va * vb);
+                    const T ex = w * (ua * vb + va * ub);
+                    const C S = tile[r][i][col];
+                    acc[u][0].x += ep * S.x;
+                    acc[u][0].y += ep * S.y;
+                    acc[u][1].x += ex * S.x;
+                    acc[u][1].y += ex * S.y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int N1, int N2, int CW, int CPR, int NT>
+__global__ __launch_bounds__(NT) void k_gw_ifft(GwFftArgs<T> A)
+{
+    using C = typename cplx_t<T>::type;
+    constexpr int N = N1 * N2, KP = N / 2 + 1;
+    constexpr int NCH = (KP + CW - 1) / CW;
+    constexpr int MPT = (N * CW + NT - 1) / NT;        /* modes owned per thread */
+    static_assert(sizeof(C) * CPR * N * (CW + 1) <= 160 * 1024, "tile exceeds the LDS of one CU");
+    __shared__ C tile[CPR][N][CW + 1];
+
+    /* XCD-aware order: adjacent kz chunks share 128-byte lines, so keep them
+     * on one XCD (one L2), working at the same time */
+    int b = blockIdx.x;
+    if (gridDim.x % 8 == 0) b = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+    const int q = b % NCH, j = b / NCH;
+    const int width = min(CW, KP - q * CW);
+    const int py = j <= N / 2 ? j : j - N;
+    const long SM = (long)N * N * A.kpad, rstride = (long)N * A.kpad;
+    const C *Sh = reinterpret_cast<const C *>(A.shat) + ((long)j * A.kpad + (long)q * CW);
+
+    /* twiddles exp(-2 pi i e/N) in the padding column of the first tile */
+    for (int e = threadIdx.x; e < N; e += NT) tile[0][e][CW] = reinterpret_cast<const C *>(A.tw)[e];
+
+    /* owned modes: e = tid + u NT -> (i = e / CW, col = e % CW) */
+    T bu[MPT][3], bv[MPT][3];
+    C acc0[MPT][2], acc1[MPT][2];
+#pragma unroll
+    for (int u = 0; u < MPT; u++) {
+        const int e = threadIdx.x + u * NT, i = e / CW, col = e % CW;
+        const int px = i <= N / 2 ? i : i - N, pz = q * CW + col;
+        const bool has = e < N * CW && col < width && (px != 0 || py != 0 || pz != 0);
+        double du[3] = {0., 0., 0.}, dv[3] = {0., 0., 0.};
+        if (has) pol_uv(px, py, pz, du, dv);
+#pragma unroll
+        for (int a = 0; a < 3; a++) { bu[u][a] = (T)du[a]; bv[u][a] = (T)dv[a]; }
+#pragma unroll
+        for (int p = 0; p < 2; p++) { acc0[u][p].x = acc0[u][p].y = (T)0; acc1[u][p].x = acc1[u][p].y = (T)0; }
+    }
+    stage_rounds<T, N1, N2, CW, CPR, NT, 0>(tile, Sh, SM, rstride, width, bu, bv, acc0);
+    stage_rounds<T, N1, N2, CW, CPR, NT, 1>(tile, Sh, SM, rstride, width, bu, bv, acc1);
+
+    /* both midpoint-RK2 stages of the owned modes (derivs, g2evolutionpert.cpp:486-501) */
+    const StepScalars *ss = A.ss;
+    const double k2u = 4. * M_PI * M_PI / A.L / A.L;
+    const T fr0 = (T)(3. * ss->adot0 / ss->a0), fr1 = (T)(3. * ss->adot1 / ss->a1);
+    const T om0 = (T)(16. * M_PI / (ss->a0 * ss->a0) * A.dx3), om1 = (T)(16. * M_PI / (ss->a1 * ss->a1) * A.dx3);
+    const double ia0 = 1. / (ss->a0 * ss->a0), ia1 = 1. / (ss->a1 * ss->a1);
+    const T hdt = (T)(0.5 * A.dt), dt = (T)A.dt;
+    C *hh = reinterpret_cast<C *>(A.h);
+    C *ll = reinterpret_cast<C *>(A.l);
+#pragma unroll
+    for (int u = 0; u < MPT; u++) {
+        const int e = threadIdx.x + u * NT, i = e / CW, col = e % CW;
+        if (e >= N * CW || col >= width) continue;
+        const int px = i <= N / 2 ? i : i - N, pz = q * CW + col;
+        const double p2 = (double)(px * px + py * py + pz * pz);
+        const T kk0 = (T)(k2u * p2 * ia0), kk1 = (T)(k2u * p2 * ia1);
+        const long m = ((long)i * N + j) * KP + pz;
+#pragma unroll
+        for (int pol = 0; pol < 2; pol++) {
+            const C h0 = hh[2 * m + pol], l0 = ll[2 * m + pol];
+            const C T0 = acc0[u][pol], T1 = acc1[u][pol];
+            C h1, l1, hn, ln;
+            h1.x = h0.x + hdt * l0.x;
+            h1.y = h0.y + hdt * l0.y;
+            l1.x = l0.x + hdt * (-kk0 * h0.x - fr0 * l0.x + om0 * T0.x);
+            l1.y = l0.y + hdt * (-kk0 * h0.y - fr0 * l0.y + om0 * T0.y);
+            hn.x = h0.x + dt * l1.x;
+            hn.y = h0.y + dt * l1.y;
+            ln.x = l0.x + dt * (-kk1 * h1.x - fr1 * l1.x + om1 * T1.x);
+            ln.y = l0.y + dt * (-kk1 * h1.y - fr1 * l1.y + om1 * T1.y);
+            hh[2 * m + pol] = hn;
+            ll[2 * m + pol] = ln;
+        }
+    }
+}
+
+template <typename T, int N1, int N2, int CW, int CPR, int NT>
+void launch_cfg(const GwFftArgs<T> &a, hipStream_t s)
+{
+    constexpr int N = N1 * N2, NCH = (N / 2 + 1 + CW - 1) / CW;
+    hipLaunchKernelGGL((k_gw_ifft<T, N1, N2, CW, CPR, NT>), dim3(N * NCH), dim3(NT), 0, s, a);
+}
+
+} // namespace
+
+bool gabe_gwfft_supported(int N, bool fp32) { return fp32 && N >= 16 && N <= 256 && (N & (N - 1)) == 0; }
+
+template <>
+void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
+{
+    switch (a.N) {
+    case 16: launch_cfg<float, 4, 4, 8, 6, 256>(a, s); break;
+    case 32: launch_cfg<float, 8, 4, 8, 6, 256>(a, s); break;
+    case 64: launch_cfg<float, 8, 8, 8, 6, 512>(a, s); break;
+    case 128: launch_cfg<float, 16, 8, 8, 6, 1024>(a, s); break;
+    case 256: launch_cfg<float, 16, 16, 8, 6, 1024>(a, s); break;
+    default: break;
+    }
+}
+
+template <>
+void launch_gw_ifft<double>(const GwFftArgs<double> &, hipStream_t) {}

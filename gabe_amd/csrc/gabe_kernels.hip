@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gabe_internal.h"
+#include "gabe_modes.h"
 
 namespace {
 
@@ -418,41 +419,6 @@ __device__ __forceinline__ void tt_project(const C S[6], R nx, R ny, R nz, R dx3
         T[c].x = (S[c].x - ni * u[ib[c]].x - u[ia[c]].x * nj + w.x * ni * nj - pij * hq.x) * dx3;
         T[c].y = (S[c].y - ni * u[ib[c]].y - u[ia[c]].y * nj + w.y * ni * nj - pij * hq.y) * dx3;
     }
-}
-
-/* TT polarisation basis of mode n = (px,py,pz) != 0: u, v orthonormal and
- * orthogonal to n (u from the coordinate axis least aligned with n, v = n x u),
- * e+ = u u - v v, ex = u v + v u, stored as components 11,12,13,22,23,33.
- * With these, Lambda_ij,lm = (e+_ij e+_lm + ex_ij ex_lm) / 2: the TT part of
- * a symmetric S is sum_pol e_pol (e_pol : S) / 2. */
-__device__ __forceinline__ void pol_basis(int px, int py, int pz, double ep[6], double ex[6])
-{
-    const double inv = 1. / sqrt((double)(px * px + py * py + pz * pz));
-    const double n[3] = {px * inv, py * inv, pz * inv};
-    int a = 0;
-    if (fabs(n[1]) < fabs(n[a])) a = 1;
-    if (fabs(n[2]) < fabs(n[a])) a = 2;
-    double u[3] = {-n[a] * n[0], -n[a] * n[1], -n[a] * n[2]};
-    u[a] += 1.;
-    const double un = 1. / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    u[0] *= un; u[1] *= un; u[2] *= un;
-    const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
-    const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-        ep[c] = u[ia[c]] * u[ib[c]] - v[ia[c]] * v[ib[c]];
-        ex[c] = u[ia[c]] * v[ib[c]] + v[ia[c]] * u[ib[c]];
-    }
-}
-
-__device__ __forceinline__ void mode_of(long m, int N, int &px, int &py, int &pz, long &row)
-{
-    const int H = N / 2 + 1;
-    pz = (int)(m % H);
-    row = m / H;
-    const int j = (int)(row % N), i = (int)(row / N);
-    px = i <= N / 2 ? i : i - N;
-    py = j <= N / 2 ? j : j - N;
 }
 
 /* One thread per mode: the TT source of both stages projected on the two

@@ -221,3 +221,23 @@ def test_gw_upload_keeps_the_tt_part():
     lat.upload_gw(trace, None)
     h2, _ = lat.download_gw()
     assert np.abs(h2).max() < 1e-13 * np.abs(junk).max()
+
+
+@pytest.mark.parametrize("N", [16, 64])
+def test_fused_gw_transform_matches_separate_passes(N, monkeypatch):
+    """FP32: the i transform fused with the mode update gives the same tensor
+    modes and fields as the separate i pass followed by the mode kernel."""
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+    o = seeded_oracle(cfg, amp=2e-3)
+    fused = lattice_from_oracle(o, cfg, GABE_FP32)
+    monkeypatch.setenv("GABE_UNFUSED_GW", "1")
+    sep = lattice_from_oracle(o, cfg, GABE_FP32)
+    monkeypatch.delenv("GABE_UNFUSED_GW")
+    fused.step(12)
+    sep.step(12)
+    hf, lf = fused.download_gw()
+    hs, ls = sep.download_gw()
+    assert rel_err(lf, ls) < 1e-5 and rel_err(hf, hs) < 1e-5
+    ff, _ = fused.download_state()
+    fs, _ = sep.download_state()
+    assert np.array_equal(ff, fs)   # the field path does not depend on the GW kernels
