@@ -26,15 +26,32 @@ METRIC = "lattice site-updates/s (RK2, 2 fields+GW) at 256³ FP32 vs FP64; % HBM
 
 
 def alg_bytes_per_site(s, N):
-    """Compulsory HBM bytes per site-update of each kernel class (s = bytes per real)."""
+    """Compulsory HBM bytes per site-update of each kernel class of this
+    pipeline (s = bytes per real, r = (N/2+1)/N modes per site).
+
+    sweep_a    reads the 4 state arrays, writes the 12 real stress arrays
+    stress_fft k-row R2C (reads 12 reals, writes 12 half-spectra) + j columns
+               (reads and writes them)
+    gw_modes   i columns fused with the mode update: reads the 12 half-spectra
+               and h, l (two TT polarisations, complex), writes h, l
+    sweep_b    reads and writes the 4 state arrays
+    """
     r = (N // 2 + 1) / N
     return {
-        "sweep_a": 16 * s,                     # read 4 state arrays, write 12 stress arrays
+        "sweep_a": 16 * s,
         "finalize": 0.0,
-        "stress_fft": 12 * s + 24 * s * r,     # 12 R2C: read reals, write half-spectra
-        "gw_modes": 72 * s * r,                # read S0,S1,h,l, write h,l (complex, per mode)
-        "sweep_b": 8 * s,                      # read 4 arrays, write 4 arrays
+        "stress_fft": 12 * s + 24 * s * r + 48 * s * r,
+        "gw_modes": 24 * s * r + 16 * s * r,
+        "sweep_b": 8 * s,
     }
+
+
+def b_alg(s, N):
+    """Bytes per site-update of the minimal dataflow used for the step
+    roofline: sweeps 12s, stress 24s, one read + write per transform 24s r,
+    six-component tensor modes 72s r (36s + 96s r)."""
+    r = (N // 2 + 1) / N
+    return 36 * s + 96 * s * r
 
 
 def read_traffic(kernel, N):
@@ -127,7 +144,7 @@ def main():
         dt64, k64, sc64, _ = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps, args.ppb)
         fp64 = {"value": sites * args.steps / dt64, "ms_per_step": 1e3 * dt64 / args.steps,
                 "kernels_ms": k64,
-                "step_hbm_frac": sum(alg_bytes_per_site(8, N).values()) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9)}
+                "step_hbm_frac": b_alg(8, N) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9)}
 
     ab = alg_bytes_per_site(4, N)
     roofline = None
@@ -164,7 +181,8 @@ def main():
                                "self-consistent expansion, FP32 fields / FP64 background, a(t) and sums",
                    "N": N, "L": cfg.L, "dt": cfg.dt, "model": "lambda phi^4/4 + g^2 phi^2 chi^2/2, g^2/lambda=100"},
         "roofline": roofline,
-        "step_hbm_frac": sum(ab.values()) * rate32 / (HBM_PEAK_GBS * 1e9),
+        "step_hbm_frac": b_alg(4, N) * rate32 / (HBM_PEAK_GBS * 1e9),
+        "step_bytes_per_site": {"b_alg": b_alg(4, N), "this_pipeline": sum(ab.values())},
         "kernels": kernels,
         "fp64": fp64,
         "cpu_baseline": cpu,

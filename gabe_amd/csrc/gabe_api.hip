@@ -85,6 +85,8 @@ struct gabe_ctx {
     bool fp32 = false, gw = false, alloc_gw = false;
     size_t esz = 8;              /* bytes per stored real */
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;      /* sweep B runs here, beside the stress transform and GW chain */
+    hipEvent_t ev_fin = nullptr, ev_side = nullptr;
     void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
     void *vbuf[2] = {nullptr, nullptr};
     int cur = 0;
@@ -192,10 +194,11 @@ SweepArgs<T> sweep_args(gabe_ctx *c)
     return a;
 }
 
-int launch_sweep_ctx(gabe_ctx *c, int mode, bool gw)
+int launch_sweep_ctx(gabe_ctx *c, int mode, bool gw, hipStream_t st = nullptr)
 {
-    if (c->fp32) launch_sweep<float>(mode, c->nf, gw, sweep_args<float>(c), c->grid, c->stream);
-    else launch_sweep<double>(mode, c->nf, gw, sweep_args<double>(c), c->grid, c->stream);
+    if (!st) st = c->stream;
+    if (c->fp32) launch_sweep<float>(mode, c->nf, gw, sweep_args<float>(c), c->grid, st);
+    else launch_sweep<double>(mode, c->nf, gw, sweep_args<double>(c), c->grid, st);
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
@@ -273,7 +276,11 @@ int gw_modes(gabe_ctx *c)
     return GABE_OK;
 }
 
-/* the kernels of one step; ev (optional) receives 2 events per kernel class */
+/* the kernels of one step; ev (optional) receives 2 events per kernel class.
+ * Without ev, sweep B (which needs only the finalize) runs on the side stream
+ * concurrently with the stress transform and the GW modes; the step ends when
+ * both are done.  With ev, the kernels run one after another so that each
+ * class is timed alone. */
 int one_step(gabe_ctx *c, hipEvent_t *ev)
 {
     auto rec = [&](int k) -> int {
@@ -286,6 +293,13 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     CHK(rec(2));
     CHK(launch_finalize_ctx(c, SW_A, 0));
     CHK(rec(3));
+    const bool overlap = c->gw && !ev && c->side;
+    if (overlap) {
+        HIPCHK(hipEventRecord(c->ev_fin, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fin, 0));
+        CHK(launch_sweep_ctx(c, SW_B, false, c->side));
+        HIPCHK(hipEventRecord(c->ev_side, c->side));
+    }
     CHK(rec(4));
     if (c->gw) CHK(stress_fft(c, !c->gwfft));
     CHK(rec(5));
@@ -293,7 +307,8 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     if (c->gw) CHK(gw_modes(c));
     CHK(rec(7));
     CHK(rec(8));
-    CHK(launch_sweep_ctx(c, SW_B, false));
+    if (overlap) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side, 0));
+    else CHK(launch_sweep_ctx(c, SW_B, false));
     CHK(rec(9));
     c->cur = 1 - c->cur;
     return GABE_OK;
@@ -353,6 +368,11 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
     c->model = gabe_model{params->m2_phi, params->lambda, params->g2, params->m2_chi};
 
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (!std::getenv("GABE_SINGLE_STREAM")) {
+        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_fin, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_side, hipEventDisableTiming));
+    }
     const size_t fbytes = c->esz * c->nf * c->N3;
     for (int s = 0; s < 2; s++) {
         HIPCHK(hipMalloc(&c->fbuf[s], fbytes));
@@ -433,6 +453,10 @@ void destroy_impl(gabe_ctx *c)
                     c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work, c->twiddle};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fin) (void)hipEventDestroy(c->ev_fin);
+    if (c->ev_side) (void)hipEventDestroy(c->ev_side);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 

@@ -13,6 +13,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "gabe_fft_dev.h"
 #include "gabe_gwfft.h"
 #include "gabe_modes.h"
@@ -239,7 +241,16 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
     case 32: launch_cfg<float, 8, 4, 8, 6, 256>(a, s); break;
     case 64: launch_cfg<float, 8, 8, 8, 6, 512>(a, s); break;
     case 128: launch_cfg<float, 16, 8, 8, 6, 1024>(a, s); break;
-    case 256: launch_cfg<float, 16, 16, 8, 6, 1024>(a, s); break;
+    case 256: {
+        /* tile variants (kz columns per workgroup, threads): see the tuning notes */
+        const char *v = std::getenv("GABE_GWFFT_TILE");
+        const int t = v ? std::atoi(v) : 0;
+        if (t == 1) launch_cfg<float, 16, 16, 4, 6, 512>(a, s);
+        else if (t == 2) launch_cfg<float, 16, 16, 4, 6, 256>(a, s);
+        else if (t == 3) launch_cfg<float, 16, 16, 8, 6, 512>(a, s);
+        else launch_cfg<float, 16, 16, 8, 6, 1024>(a, s);
+        break;
+    }
     default: break;
     }
 }
