@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "gabe.h"
@@ -85,6 +86,7 @@ struct gabe_ctx {
     bool fp32 = false, gw = false, alloc_gw = false;
     size_t esz = 8;              /* bytes per stored real */
     hipStream_t stream = nullptr;
+    bool own_stream = true;          /* false for ensemble members (the ensemble's stream) */
     hipStream_t side = nullptr;      /* sweep B runs here, beside the stress transform and GW chain */
     hipEvent_t ev_fin = nullptr, ev_side = nullptr;
     void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
@@ -187,6 +189,8 @@ SweepArgs<T> sweep_args(gabe_ctx *c)
     a.N = c->N;
     a.ci = c->ci;
     a.nblk = c->nblk;
+    a.gx = (int)c->grid.x;
+    a.gy = (int)c->grid.y;
     a.N2 = c->N2;
     a.N3 = c->N3;
     a.dt = c->p.dt;
@@ -340,6 +344,17 @@ int refresh_e0(gabe_ctx *c)
     return GABE_OK;
 }
 
+/* sweep tiling chosen by gabe_autotune_sweep, keyed by (N, nflds, FP32) */
+std::map<long, int> g_tuned;
+
+long tune_key(int N, int nf, bool fp32) { return ((long)N * 4 + nf) * 2 + (fp32 ? 1 : 0); }
+
+int tuned_planes(int N, int nf, bool fp32)
+{
+    const auto it = g_tuned.find(tune_key(N, nf, fp32));
+    return it == g_tuned.end() ? 0 : it->second;
+}
+
 int check_params(const gabe_params *p)
 {
     if (!p) return fail(GABE_ERR_ARG, "params is NULL");
@@ -352,7 +367,7 @@ int check_params(const gabe_params *p)
     return GABE_OK;
 }
 
-int create_impl(const gabe_params *params, gabe_ctx *c)
+int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nullptr)
 {
     c->p = *params;
     c->N = params->N;
@@ -367,8 +382,13 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
     c->gw = params->calc_gws && params->expansion_type == 1; /* Q15: only type 1 evolves GWs */
     c->model = gabe_model{params->m2_phi, params->lambda, params->g2, params->m2_chi};
 
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    if (!std::getenv("GABE_SINGLE_STREAM")) {
+    if (shared) {
+        c->stream = shared;
+        c->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
+    if (!shared && !std::getenv("GABE_SINGLE_STREAM")) {
         HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&c->ev_fin, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->ev_side, hipEventDisableTiming));
@@ -397,6 +417,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c)
     /* sweep tiling: one thread per plane column, ci planes per workgroup */
     const int bx = (int)((c->N2 + GABE_BLOCK - 1) / GABE_BLOCK);
     int ci = params->planes_per_block;
+    if (ci <= 0) ci = tuned_planes(c->N, c->nf, c->fp32);
     if (ci <= 0) {
         const int by_target = std::max(1, 2048 / bx);
         ci = (c->N + by_target - 1) / by_target;
@@ -457,7 +478,7 @@ void destroy_impl(gabe_ctx *c)
     if (c->ev_fin) (void)hipEventDestroy(c->ev_fin);
     if (c->ev_side) (void)hipEventDestroy(c->ev_side);
     if (c->side) (void)hipStreamDestroy(c->side);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
 }
 
 /* host [nf][N3] doubles -> device storage of slot cur (split into mean + fluctuation for FP32) */
@@ -867,6 +888,254 @@ int gabe_kernel_times(gabe_ctx *c, int nsteps, double *ms_out)
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+} // extern "C"
+
+struct gabe_ensemble {
+    int count = 0, nf = 0;
+    bool fp32 = false;
+    hipStream_t stream = nullptr;
+    std::vector<gabe_ctx *> members;
+    void *d_desc[2] = {nullptr, nullptr};   /* SweepArgs<T>[count] for state parity 0 / 1 */
+    FinArgs *d_fin = nullptr;
+    int *d_start = nullptr;
+    int nblk = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   /* two steps starting at parity 0 / 1 */
+};
+
+namespace {
+
+int ens_launch_step(gabe_ensemble *e, int parity)
+{
+    if (e->fp32) {
+        const auto *d = (const SweepArgs<float> *)e->d_desc[parity];
+        launch_sweep_batch<float>(SW_A, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
+        launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
+        launch_sweep_batch<float>(SW_B, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
+    } else {
+        const auto *d = (const SweepArgs<double> *)e->d_desc[parity];
+        launch_sweep_batch<double>(SW_A, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
+        launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
+        launch_sweep_batch<double>(SW_B, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+template <typename T>
+int ens_upload_desc(gabe_ensemble *e)
+{
+    for (int p = 0; p < 2; p++) {
+        std::vector<SweepArgs<T>> h(e->count);
+        for (int m = 0; m < e->count; m++) {
+            gabe_ctx *c = e->members[m];
+            const int keep = c->cur;
+            c->cur = p;
+            h[m] = sweep_args<T>(c);
+            c->cur = keep;
+        }
+        HIPCHK(hipMalloc(&e->d_desc[p], sizeof(SweepArgs<T>) * e->count));
+        HIPCHK(hipMemcpy(e->d_desc[p], h.data(), sizeof(SweepArgs<T>) * e->count, hipMemcpyHostToDevice));
+    }
+    return GABE_OK;
+}
+
+void ens_free(gabe_ensemble *e)
+{
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &g : e->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (gabe_ctx *c : e->members) {
+        if (!c) continue;
+        destroy_impl(c);
+        delete c;
+    }
+    void *bufs[] = {e->d_desc[0], e->d_desc[1], e->d_fin, e->d_start};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+}
+
+int ens_create(const gabe_params *params, int count, gabe_ensemble *e)
+{
+    e->count = count;
+    e->nf = params[0].nflds;
+    e->fp32 = params[0].precision == GABE_FP32;
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    std::vector<int> start(count + 1, 0);
+    std::vector<FinArgs> fin(count);
+    for (int m = 0; m < count; m++) {
+        gabe_ctx *c = new gabe_ctx;
+        e->members.push_back(c);
+        CHK(create_impl(&params[m], c, e->stream));
+        const int nb = (int)(c->grid.x * c->grid.y);
+        start[m + 1] = start[m] + (nb + GABE_NXCD - 1) / GABE_NXCD * GABE_NXCD;
+        FinArgs &F = fin[m];
+        F = FinArgs{};
+        F.partials = c->partials;
+        F.nblk = c->nblk;
+        F.nf = c->nf;
+        F.mode = SW_A;
+        F.exp_type = c->p.expansion_type;
+        F.split = c->fp32 ? 1 : 0;
+        F.N3 = (double)c->N3;
+        F.dt = c->p.dt;
+        F.sc = c->sc;
+        F.ss = c->ss;
+        F.hist = c->hist;
+        F.hist_cap = c->hist_cap;
+    }
+    e->nblk = start[count];
+    HIPCHK(hipMalloc(&e->d_start, sizeof(int) * (count + 1)));
+    HIPCHK(hipMemcpy(e->d_start, start.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&e->d_fin, sizeof(FinArgs) * count));
+    HIPCHK(hipMemcpy(e->d_fin, fin.data(), sizeof(FinArgs) * count, hipMemcpyHostToDevice));
+    if (e->fp32) CHK(ens_upload_desc<float>(e));
+    else CHK(ens_upload_desc<double>(e));
+    return GABE_OK;
+}
+
+/* time `steps` steps of one lattice with HIP events */
+int time_steps(gabe_ctx *c, int steps, float &ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = GABE_OK;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GABE_ERR_HIP, "event record failed");
+    for (int n = 0; n < steps && rc == GABE_OK; n++) rc = one_step(c, nullptr);
+    if (rc == GABE_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = fail(GABE_ERR_HIP, "event sync failed");
+    if (rc == GABE_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int gabe_query(gabe_ctx *c, gabe_params *out)
+{
+    if (!c || !out) return fail(GABE_ERR_ARG, "NULL argument");
+    *out = c->p;
+    out->planes_per_block = c->ci;
+    return GABE_OK;
+}
+
+int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double *ms_per_step)
+{
+    CHK(check_params(params));
+    gabe_params p = *params;
+    const int cands[] = {1, 2, 4, 8, 16, 32, 64};
+    int best = 0;
+    float best_ms = 0.f;
+    for (int ci : cands) {
+        if (ci > p.N) break;
+        p.planes_per_block = ci;
+        gabe_ctx *c = new gabe_ctx;
+        int rc = create_impl(&p, c);
+        if (rc == GABE_OK) {
+            /* a smooth synthetic state: timings do not depend on the values */
+            const long n = (long)p.nflds * c->N3;
+            std::vector<double> f(n), d(n);
+            for (long q = 0; q < n; q++) {
+                f[q] = 0.3 + 1e-6 * std::sin(0.01 * (double)q);
+                d[q] = -0.2 + 1e-6 * std::cos(0.013 * (double)q);
+            }
+            rc = gabe_upload_state(c, f.data(), d.data());
+            float ms = 0.f;
+            if (rc == GABE_OK) rc = time_steps(c, 3, ms);
+            if (rc == GABE_OK) rc = time_steps(c, 10, ms);
+            if (rc == GABE_OK && (best == 0 || ms < best_ms)) {
+                best = ci;
+                best_ms = ms;
+            }
+        }
+        destroy_impl(c);
+        delete c;
+        if (rc != GABE_OK) return rc;
+    }
+    g_tuned[tune_key(p.N, p.nflds, p.precision == GABE_FP32)] = best;
+    if (planes_per_block) *planes_per_block = best;
+    if (ms_per_step) *ms_per_step = best_ms / 10.;
+    return GABE_OK;
+}
+
+int gabe_ensemble_create(const gabe_params *params, int count, gabe_ensemble **out)
+{
+    if (!out || !params || count <= 0) return fail(GABE_ERR_ARG, "bad argument");
+    *out = nullptr;
+    for (int m = 0; m < count; m++) {
+        CHK(check_params(&params[m]));
+        if (params[m].calc_gws) return fail(GABE_ERR_ARG, "ensemble member %d: tensor modes are not batched", m);
+        if (params[m].precision != params[0].precision || params[m].nflds != params[0].nflds)
+            return fail(GABE_ERR_ARG, "ensemble members must share precision and nflds");
+    }
+    gabe_ensemble *e = new gabe_ensemble;
+    const int rc = ens_create(params, count, e);
+    if (rc != GABE_OK) {
+        const std::string keep = g_err;
+        ens_free(e);
+        delete e;
+        g_err = keep;
+        return rc;
+    }
+    *out = e;
+    return GABE_OK;
+}
+
+int gabe_ensemble_member(gabe_ensemble *e, int idx, gabe_ctx **out)
+{
+    if (!e || !out || idx < 0 || idx >= e->count) return fail(GABE_ERR_ARG, "bad member index");
+    *out = e->members[idx];
+    return GABE_OK;
+}
+
+int gabe_ensemble_step(gabe_ensemble *e, int nsteps)
+{
+    if (!e || nsteps < 0) return fail(GABE_ERR_ARG, "bad argument");
+    const int parity = e->members[0]->cur;
+    for (gabe_ctx *c : e->members)
+        if (c->cur != parity) return fail(GABE_ERR_STATE, "ensemble members were stepped separately by an odd count");
+    int p = parity, n = nsteps;
+    if (n >= 2 && !e->graph[p]) {   /* capture two steps: the state returns to parity p */
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = ens_launch_step(e, p);
+        if (rc == GABE_OK) rc = ens_launch_step(e, 1 - p);
+        const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+        if (rc != GABE_OK) return rc;
+        if (ec != hipSuccess) return fail(GABE_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&e->graph[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(GABE_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    for (; n >= 2; n -= 2) HIPCHK(hipGraphLaunch(e->graph[p], e->stream));
+    if (n == 1) {
+        CHK(ens_launch_step(e, p));
+        p = 1 - p;
+    }
+    for (gabe_ctx *c : e->members) c->cur = p;
+    return GABE_OK;
+}
+
+int gabe_ensemble_sync(gabe_ensemble *e)
+{
+    if (!e) return fail(GABE_ERR_ARG, "NULL ensemble");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (gabe_ctx *c : e->members) CHK(gabe_sync(c));
+    return GABE_OK;
+}
+
+int gabe_ensemble_destroy(gabe_ensemble *e)
+{
+    if (!e) return GABE_OK;
+    ens_free(e);
+    delete e;
+    return GABE_OK;
 }
 
 } // extern "C"

@@ -59,6 +59,7 @@ struct SweepArgs {
     const StepScalars *ss;
     gabe_model model;
     int N, ci, nblk;
+    int gx, gy;             /* workgroup grid of this lattice's sweeps */
     long N2, N3;
     double dt, inv_dx2;
 };
@@ -97,6 +98,10 @@ struct SpecArgs {
 template <typename T>
 void launch_sweep(int mode, int nf, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s);
 void launch_finalize(const FinArgs &a, hipStream_t s);
+template <typename T>
+void launch_sweep_batch(int mode, int nf, const SweepArgs<T> *descs, const int *start, int count, int nblk,
+                        hipStream_t s);
+void launch_finalize_batch(const FinArgs *descs, int nf, int count, hipStream_t s);
 template <typename T>
 void launch_gw(const GwArgs<T> &a, hipStream_t s);
 template <typename T>

@@ -69,7 +69,7 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
  * centre loads of the adjacent lanes and rows).  Consecutive lanes read
  * consecutive k: every load is coalesced. */
 template <typename T, int NF, int MODE, bool GW>
-__global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
+__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy)
 {
     constexpr int NS = gabe_nsums(NF);
     const int N = A.N;
@@ -78,13 +78,13 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
      * the 8 XCDs, each with its own L2.  Give every XCD a contiguous range of
      * rows instead, so the j+-1 neighbour rows a workgroup reads are being
      * read by workgroups on the same XCD at the same time. */
-    int bx = blockIdx.x, by = blockIdx.y;
+    int bx = hw % gx, by = hw / gx;
     {
-        const int nb = gridDim.x * gridDim.y, hw = blockIdx.y * gridDim.x + blockIdx.x;
+        const int nb = gx * gy;
         if (nb % GABE_NXCD == 0) {
             const int logical = (hw % GABE_NXCD) * (nb / GABE_NXCD) + hw / GABE_NXCD;
-            bx = logical % gridDim.x;
-            by = logical / gridDim.x;
+            bx = logical % gx;
+            by = logical / gx;
         }
     }
     const long p = (long)bx * GABE_BLOCK + threadIdx.x;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
     const long pjm = (long)(j == 0 ? N - 1 : j - 1) * N + k;
     const int i0 = by * A.ci;
     const int i1 = min(i0 + A.ci, N);
-    const int bid = by * gridDim.x + bx;
+    const int bid = by * gx + bx;
 
     const T idx2 = (T)A.inv_dx2;
     const double dt = A.dt, hdt = 0.5 * dt;
@@ -245,6 +245,31 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
     if (MODE != SW_B) block_store_partials<NS>(acc, A.partials, A.nblk, bid);
 }
 
+template <typename T, int NF, int MODE, bool GW>
+__global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
+{
+    sweep_body<T, NF, MODE, GW>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y);
+}
+
+/* every lattice of an ensemble in one launch: workgroups [start[L], start[L+1])
+ * run lattice L with the block decomposition of its own single launch (each
+ * start is a multiple of the XCD count, so the XCD-aware order is unchanged) */
+template <typename T, int NF, int MODE, bool GW>
+__global__ __launch_bounds__(GABE_BLOCK) void k_sweep_batch(const SweepArgs<T> *descs, const int *start, int count)
+{
+    int lo = 0, hi = count - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const SweepArgs<T> &A = descs[lo];
+    const int hw = b - start[lo];
+    if (hw >= A.gx * A.gy) return;   /* padding workgroup */
+    sweep_body<T, NF, MODE, GW>(A, hw, A.gx, A.gy);
+}
+
 /* ---------------------------------------------------------------- finalize */
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)
 {
@@ -258,7 +283,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
  * update of the FP32 form.  Deterministic: the same partials give the same
  * bits on every run. */
 template <int NF>
-__global__ __launch_bounds__(GABE_BLOCK) void k_finalize(FinArgs F)
+__device__ __forceinline__ void finalize_body(const FinArgs &F)
 {
     constexpr int NS = gabe_nsums(NF);
     double s[NS], c[NS];
@@ -382,6 +407,19 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_finalize(FinArgs F)
     sc->adot0_valid = 0;
     sc->e0_valid = 0;
     if (!isfinite(edrho0) || !isfinite(edrho1)) sc->error = 1;
+}
+
+template <int NF>
+__global__ __launch_bounds__(GABE_BLOCK) void k_finalize(FinArgs F)
+{
+    finalize_body<NF>(F);
+}
+
+/* one workgroup per lattice of an ensemble */
+template <int NF>
+__global__ __launch_bounds__(GABE_BLOCK) void k_finalize_batch(const FinArgs *descs)
+{
+    finalize_body<NF>(descs[blockIdx.x]);
 }
 
 /* ------------------------------------------------------------- GW modes */
@@ -677,6 +715,31 @@ static void launch_sweep_nf(int mode, bool gw, const SweepArgs<T> &a, dim3 grid,
     else hipLaunchKernelGGL((k_sweep<T, NF, SW_B, false>), grid, dim3(GABE_BLOCK), 0, s, a);
 }
 
+template <typename T, int NF>
+static void launch_sweep_batch_nf(int mode, const SweepArgs<T> *d, const int *start, int count, int nblk,
+                                  hipStream_t s)
+{
+    if (mode == SW_ENERGY)
+        hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_ENERGY, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
+    else if (mode == SW_A)
+        hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_A, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
+    else hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_B, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
+}
+
+template <typename T>
+void launch_sweep_batch(int mode, int nf, const SweepArgs<T> *descs, const int *start, int count, int nblk,
+                        hipStream_t s)
+{
+    if (nf == 1) launch_sweep_batch_nf<T, 1>(mode, descs, start, count, nblk, s);
+    else launch_sweep_batch_nf<T, 2>(mode, descs, start, count, nblk, s);
+}
+
+void launch_finalize_batch(const FinArgs *descs, int nf, int count, hipStream_t s)
+{
+    if (nf == 1) hipLaunchKernelGGL(k_finalize_batch<1>, dim3(count), dim3(GABE_BLOCK), 0, s, descs);
+    else hipLaunchKernelGGL(k_finalize_batch<2>, dim3(count), dim3(GABE_BLOCK), 0, s, descs);
+}
+
 template <typename T>
 void launch_sweep(int mode, int nf, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s)
 {
@@ -741,6 +804,7 @@ void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
 
 #define GABE_INST(T)                                                                                          \
     template void launch_sweep<T>(int, int, bool, const SweepArgs<T> &, dim3, hipStream_t);                   \
+    template void launch_sweep_batch<T>(int, int, const SweepArgs<T> *, const int *, int, int, hipStream_t);  \
     template void launch_gw<T>(const GwArgs<T> &, hipStream_t);                                               \
     template void launch_gw_pack<T>(const double *, T *, int, long, hipStream_t);                            \
     template void launch_gw_unpack<T>(const T *, double *, int, long, hipStream_t);                          \

@@ -54,6 +54,11 @@ def friedmann_residual(hist):
     return worst, out
 
 
+def initialize_member(lat, cfg, rand_init=1):
+    """Reference start-up sequence on an existing lattice (e.g. an ensemble member)."""
+    initialize(lat, cfg, rand_init)
+
+
 def initialized_lattice(cfg, precision=_lat.GABE_FP32, **kw):
     lat = _lat.Lattice(precision=precision, **cfg.lattice_kwargs(), **kw)
     initialize(lat, cfg)

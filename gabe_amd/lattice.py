@@ -29,7 +29,9 @@ EXPORTS = (
     "gabe_upload_state", "gabe_download_state", "gabe_upload_gw", "gabe_download_gw",
     "gabe_set_scalars", "gabe_get_scalars", "gabe_energy", "gabe_step", "gabe_sync",
     "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
-    "gabe_history", "gabe_kernel_times",
+    "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep",
+    "gabe_ensemble_create", "gabe_ensemble_member", "gabe_ensemble_step", "gabe_ensemble_sync",
+    "gabe_ensemble_destroy",
 )
 
 
@@ -87,6 +89,13 @@ def load():
         "gabe_tt_source": [vp, ip, dp],
         "gabe_history": [vp, dp, ip],
         "gabe_kernel_times": [vp, ip, dp],
+        "gabe_query": [vp, C.POINTER(GabeParams)],
+        "gabe_autotune_sweep": [C.POINTER(GabeParams), C.POINTER(C.c_int), dp],
+        "gabe_ensemble_create": [C.POINTER(GabeParams), ip, C.POINTER(vp)],
+        "gabe_ensemble_member": [vp, ip, C.POINTER(vp)],
+        "gabe_ensemble_step": [vp, ip],
+        "gabe_ensemble_sync": [vp],
+        "gabe_ensemble_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -112,6 +121,24 @@ def numbins(N):
     return load().gabe_numbins(N)
 
 
+def make_params(N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1, precision=GABE_FP64, m2_phi=1.0,
+                lam=0.0, g2=0.0, m2_chi=0.0, gw_spectrum_quirk=1, planes_per_block=0):
+    if dt is None:
+        dt = L / N / 20.0
+    return GabeParams(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
+                      gw_spectrum_quirk, planes_per_block)
+
+
+def autotune_sweep(**kw):
+    """Time the candidate sweep tilings for these parameters; later lattices
+    of the same (N, nflds, precision) use the fastest.  Returns (ppb, ms)."""
+    p = make_params(**kw)
+    ppb = C.c_int()
+    ms = C.c_double()
+    _check(load().gabe_autotune_sweep(C.byref(p), C.byref(ppb), C.byref(ms)))
+    return ppb.value, ms.value
+
+
 class Lattice:
     """One device-resident lattice (a gabe_ctx).
 
@@ -121,21 +148,23 @@ class Lattice:
 
     def __init__(self, N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1,
                  precision=GABE_FP64, m2_phi=1.0, lam=0.0, g2=0.0, m2_chi=0.0,
-                 gw_spectrum_quirk=1, planes_per_block=0):
-        if dt is None:
-            dt = L / N / 20.0
-        self.params = GabeParams(N, nflds, L, dt, calc_gws, expansion_type, precision,
-                                 m2_phi, lam, g2, m2_chi, gw_spectrum_quirk, planes_per_block)
+                 gw_spectrum_quirk=1, planes_per_block=0, _handle=None):
+        self.params = make_params(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
+                                  gw_spectrum_quirk, planes_per_block)
         self.N, self.nflds, self.calc_gws = N, nflds, calc_gws
         self.M = N * N * (N // 2 + 1)
+        self._owned = _handle is None
+        if _handle is not None:
+            self._h = _handle
+            return
         h = C.c_void_p()
         _check(load().gabe_create(C.byref(self.params), C.byref(h)))
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None) and self._h.value:
+        if getattr(self, "_owned", True) and getattr(self, "_h", None) and self._h.value:
             load().gabe_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -232,3 +261,43 @@ class Lattice:
         out = np.zeros(GABE_NKCLASS)
         _check(load().gabe_kernel_times(self._h, nsteps, _dp(out)))
         return dict(zip(KERNEL_CLASSES, out.tolist()))
+
+    def query(self):
+        p = GabeParams()
+        _check(load().gabe_query(self._h, C.byref(p)))
+        return p
+
+
+class Ensemble:
+    """Independent lattices stepped together (gabe_ensemble_*).  members[i] is
+    a Lattice view of member i, usable with every per-lattice call."""
+
+    def __init__(self, member_kwargs):
+        arr = (GabeParams * len(member_kwargs))(*[make_params(**kw) for kw in member_kwargs])
+        h = C.c_void_p()
+        _check(load().gabe_ensemble_create(arr, len(member_kwargs), C.byref(h)))
+        self._h = h
+        self.members = []
+        for i, kw in enumerate(member_kwargs):
+            mh = C.c_void_p()
+            _check(load().gabe_ensemble_member(self._h, i, C.byref(mh)))
+            self.members.append(Lattice(_handle=mh, **kw))
+
+    def step(self, nsteps=1):
+        _check(load().gabe_ensemble_step(self._h, nsteps))
+
+    def sync(self):
+        _check(load().gabe_ensemble_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            for m in self.members:
+                m._h = C.c_void_p()
+            load().gabe_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

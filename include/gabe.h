@@ -150,6 +150,30 @@ int gabe_history(gabe_ctx *ctx, double *out, int max_rows);
 #define GABE_NKCLASS 5
 int gabe_kernel_times(gabe_ctx *ctx, int nsteps, double *ms_out);
 
+/* Effective parameters of a context, with the sweep tiling actually used
+ * in planes_per_block.  No reference counterpart (introspection). */
+int gabe_query(gabe_ctx *ctx, gabe_params *out);
+
+/* Sweep-tiling autotuner: times a few steps of a synthetic lattice of these
+ * parameters for each candidate planes_per_block and remembers the fastest
+ * for (N, nflds, precision); later contexts created with planes_per_block = 0
+ * use it.  Either output pointer may be NULL.  No reference counterpart. */
+int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double *ms_per_step);
+
+/* Ensembles: independent lattices (sizes, couplings, seeds and a(t) of their
+ * own) advanced together, each RK2 phase of every member in one launch and
+ * two steps per captured HIP graph.  Members are ordinary contexts for every
+ * other call (upload, initialise, scalars, spectra, download) and give
+ * bit-identical results to the same lattice stepped alone.  All members share
+ * precision and nflds; tensor modes are not batched (calc_gws must be 0).
+ * No reference counterpart: the reference runs one lattice per process. */
+typedef struct gabe_ensemble gabe_ensemble;
+int gabe_ensemble_create(const gabe_params *params, int count, gabe_ensemble **out);
+int gabe_ensemble_member(gabe_ensemble *ens, int idx, gabe_ctx **out);
+int gabe_ensemble_step(gabe_ensemble *ens, int nsteps);
+int gabe_ensemble_sync(gabe_ensemble *ens);
+int gabe_ensemble_destroy(gabe_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif
