@@ -86,6 +86,78 @@ def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
     return t1 - t0, ktimes, sc, state
 
 
+def ensemble_members(sizes=(32, 64, 96, 128), per_size=16, seed=20240):
+    """64 independent lattices: 16 each of 32^3, 64^3, 96^3, 128^3 with L = 20,
+    dt = dx/20, distinct IC seeds and g^2/lambda log-uniform in [1, 200]; no
+    tensor modes (the batched path steps fields and expansion only)."""
+    import math
+    import random
+    from gabe_amd import configs
+    rng = random.Random(seed)
+    cfgs = []
+    for N in sizes:
+        for _ in range(per_size):
+            ratio = math.exp(rng.uniform(0.0, math.log(200.0)))
+            c = configs.lambda_phi4_config(N=N, calc_gws=0, g2_over_lambda=ratio)
+            c.randseed = 1 + len(cfgs) * 7919
+            cfgs.append(c)
+    return cfgs
+
+
+def run_ensemble(steps, warmup):
+    """Aggregate site-updates/s of the batched ensemble against the same 64
+    lattices run one after another, each to completion, on their own contexts."""
+    from gabe_amd import host
+    from gabe_amd.lattice import GABE_FP32, Ensemble, Lattice, autotune_sweep
+    cfgs = ensemble_members()
+    sites = sum(c.N ** 3 for c in cfgs)
+    tuned = {}
+    for c in cfgs:
+        if c.N not in tuned:
+            tuned[c.N] = autotune_sweep(precision=GABE_FP32, **c.lattice_kwargs())[0]
+    ens = Ensemble([dict(precision=GABE_FP32, **c.lattice_kwargs()) for c in cfgs])
+    for c, m in zip(cfgs, ens.members):
+        host.initialize_member(m, c)
+    ens.step(warmup)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    t_ens = time.perf_counter() - t0
+    ppb = [m.query().planes_per_block for m in ens.members]
+    ens.close()
+
+    singles = []
+    for c, p in zip(cfgs, ppb):
+        s = Lattice(precision=GABE_FP32, planes_per_block=p, **c.lattice_kwargs())
+        host.initialize(s, c)
+        s.step(warmup)
+        s.sync()
+        singles.append(s)
+    t0 = time.perf_counter()
+    for s in singles:
+        s.step(steps)
+        s.sync()
+    t_seq = time.perf_counter() - t0
+    for s in singles:
+        s.close()
+    return {
+        "value": sites * steps / t_ens,
+        "unit": "site-updates/s",
+        "sequential_value": sites * steps / t_seq,
+        "speedup_vs_sequential": t_seq / t_ens,
+        "ms_per_step": 1e3 * t_ens / steps,
+        "sequential_ms_per_step": 1e3 * t_seq / steps,
+        "autotuned_planes_per_block": {str(n): p for n, p in tuned.items()},
+        "lattices": len(cfgs),
+        "sites": sites,
+        "steps": steps,
+        "warmup": warmup,
+        "workload": "64 lattices (16 each of 32^3, 64^3, 96^3, 128^3), L=20, dt=dx/20, 2 fields, no GW, "
+                    "distinct seeds, g^2/lambda log-uniform in [1,200], FP32 fields / FP64 per-lattice a(t)",
+    }
+
+
 def cpu_baseline(cfg, state, nsteps=2):
     """The CPU FP64 oracle, OpenMP over the outer index, on this host's cores."""
     from oracle import pyoracle as po
@@ -121,6 +193,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-lattice ensemble (0: skip)")
     ap.add_argument("--ppb", type=int, default=0, help="lattice planes per sweep workgroup (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -158,6 +231,11 @@ def main():
     kernels = {k: {"ms": v, "alg_GBs": (ab[k] * sites / (v * 1e-3) / 1e9) if v > 0 and ab[k] else None}
                for k, v in k32.items()}
 
+    ensemble = None
+    if args.ensemble_steps:
+        print(f"[bench] ensemble of 64 lattices, {args.ensemble_steps} steps", file=sys.stderr, flush=True)
+        ensemble = run_ensemble(args.ensemble_steps, min(args.warmup, 20))
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
@@ -185,6 +263,7 @@ def main():
         "step_bytes_per_site": {"b_alg": b_alg(4, N), "this_pipeline": sum(ab.values())},
         "kernels": kernels,
         "fp64": fp64,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
         "final_a": sc32["a"][0],
     }

@@ -31,6 +31,9 @@ class ModelConfig:
     randseed: int = 44463132           # g2parameters.h:56
     calc_gws: int = 1
     expansion_type: int = 1
+    field_full_rand: int = 1           # g2parameters.h:72
+    spec_cut_off: float = 0.0          # g2parameters.h:73 (0: not defined)
+    spec_smooth: float = 0.5           # g2parameters.h:74
 
     @property
     def dx(self):

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "gabe.h"
@@ -346,11 +347,13 @@ int refresh_e0(gabe_ctx *c)
 
 /* sweep tiling chosen by gabe_autotune_sweep, keyed by (N, nflds, FP32) */
 std::map<long, int> g_tuned;
+std::mutex g_tuned_mu;
 
 long tune_key(int N, int nf, bool fp32) { return ((long)N * 4 + nf) * 2 + (fp32 ? 1 : 0); }
 
 int tuned_planes(int N, int nf, bool fp32)
 {
+    std::lock_guard<std::mutex> lock(g_tuned_mu);
     const auto it = g_tuned.find(tune_key(N, nf, fp32));
     return it == g_tuned.end() ? 0 : it->second;
 }
@@ -571,6 +574,66 @@ int gabe_download_state(gabe_ctx *c, double *field, double *dfield)
     CHK(read_scalars(c, d));
     if (field) CHK(download_fields(c, c->fbuf[c->cur], field, d.pbar));
     if (dfield) CHK(download_fields(c, c->vbuf[c->cur], dfield, d.vbar));
+    return GABE_OK;
+}
+
+int gabe_download_plane(gabe_ctx *c, int i, double *field, double *dfield)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    if (i < 0 || i >= c->N) return fail(GABE_ERR_ARG, "plane index %d outside [0, %d)", i, c->N);
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    const long N2 = c->N2, off = (long)i * N2;
+    const void *src[2] = {c->fbuf[c->cur], c->vbuf[c->cur]};
+    double *dst[2] = {field, dfield};
+    const double *means[2] = {d.pbar, d.vbar};
+    std::vector<float> tmp(c->fp32 ? N2 : 0);
+    for (int a = 0; a < 2; a++) {
+        if (!dst[a]) continue;
+        for (int f = 0; f < c->nf; f++) {
+            double *out = dst[a] + (long)f * N2;
+            if (c->fp32) {
+                HIPCHK(hipMemcpy(tmp.data(), (const float *)src[a] + (long)f * c->N3 + off, sizeof(float) * N2,
+                                 hipMemcpyDeviceToHost));
+                for (long n = 0; n < N2; n++) out[n] = means[a][f] + (double)tmp[n];
+            } else {
+                HIPCHK(hipMemcpy(out, (const double *)src[a] + (long)f * c->N3 + off, sizeof(double) * N2,
+                                 hipMemcpyDeviceToHost));
+            }
+        }
+    }
+    return GABE_OK;
+}
+
+int gabe_means_vars(gabe_ctx *c, double *out)
+{
+    if (!c || !out) return fail(GABE_ERR_ARG, "NULL argument");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    const int nblk = 1024;
+    const long N3 = c->N3;
+    std::vector<double> part(2 * nblk);
+    for (int f = 0; f < c->nf; f++) {
+        double shift = 0.;
+        float shift32 = 0.f;
+        if (c->fp32) {
+            const float *x = (const float *)c->fbuf[c->cur] + (long)f * N3;
+            launch_meanvar<float>(x, N3, c->partials, nblk, c->stream);
+            HIPCHK(hipMemcpyAsync(&shift32, x, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        } else {
+            const double *x = (const double *)c->fbuf[c->cur] + (long)f * N3;
+            launch_meanvar<double>(x, N3, c->partials, nblk, c->stream);
+            HIPCHK(hipMemcpyAsync(&shift, x, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(part.data(), c->partials, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->fp32) shift = (double)shift32;
+        const double m = csum(part.data(), nblk) / (double)N3;
+        const double base = c->fp32 ? d.pbar[f] : 0.;
+        out[2 * f] = base + shift + m;
+        out[2 * f + 1] = csum(part.data() + nblk, nblk) / (double)N3 - m * m;
+    }
     return GABE_OK;
 }
 
@@ -1058,7 +1121,10 @@ int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double
         delete c;
         if (rc != GABE_OK) return rc;
     }
-    g_tuned[tune_key(p.N, p.nflds, p.precision == GABE_FP32)] = best;
+    {
+        std::lock_guard<std::mutex> lock(g_tuned_mu);
+        g_tuned[tune_key(p.N, p.nflds, p.precision == GABE_FP32)] = best;
+    }
     if (planes_per_block) *planes_per_block = best;
     if (ms_per_step) *ms_per_step = best_ms / 10.;
     return GABE_OK;

@@ -116,6 +116,8 @@ template <typename T>
 void launch_effmass(const T *f0, const T *f1, int nf, int fld, double pbar0, double pbar1,
                     gabe_model m, long n3, double *partials, int nblk, hipStream_t s);
 template <typename T>
+void launch_meanvar(const T *f, long n3, double *partials, int nblk, hipStream_t s);
+template <typename T>
 void launch_add_real(T *dst, const double *add, long n, hipStream_t s);
 
 #endif

@@ -696,6 +696,21 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_effmass(const T *f0, const T *f1
     block_store_partials<1>(acc, partials, gridDim.x, blockIdx.x);
 }
 
+/* shifted one-pass mean/variance partials of one field: sums of d and d^2 with
+ * d = f - f[0], so the variance does not cancel against the squared mean */
+template <typename T>
+__global__ __launch_bounds__(GABE_BLOCK) void k_meanvar(const T *f, long n3, double *partials)
+{
+    const double shift = (double)f[0];
+    double acc[2] = {0., 0.};
+    for (long n = (long)blockIdx.x * GABE_BLOCK + threadIdx.x; n < n3; n += (long)gridDim.x * GABE_BLOCK) {
+        const double d = (double)f[n] - shift;
+        acc[0] += d;
+        acc[1] += d * d;
+    }
+    block_store_partials<2>(acc, partials, gridDim.x, blockIdx.x);
+}
+
 template <typename T>
 __global__ __launch_bounds__(GABE_BLOCK) void k_add_real(T *dst, const double *add, long n)
 {
@@ -796,6 +811,12 @@ void launch_effmass(const T *f0, const T *f1, int nf, int fld, double pbar0, dou
 }
 
 template <typename T>
+void launch_meanvar(const T *f, long n3, double *partials, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_meanvar<T>), dim3(nblk), dim3(GABE_BLOCK), 0, s, f, n3, partials);
+}
+
+template <typename T>
 void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
 {
     const unsigned nb = (unsigned)((n + GABE_BLOCK - 1) / GABE_BLOCK);
@@ -812,6 +833,7 @@ void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
     template void launch_spectra<T>(const SpecArgs<T> &, hipStream_t);                                        \
     template void launch_effmass<T>(const T *, const T *, int, int, double, double, gabe_model, long,        \
                                     double *, int, hipStream_t);                                              \
+    template void launch_meanvar<T>(const T *, long, double *, int, hipStream_t);                            \
     template void launch_add_real<T>(T *, const double *, long, hipStream_t);
 GABE_INST(float)
 GABE_INST(double)

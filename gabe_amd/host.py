@@ -11,7 +11,8 @@ LIB_PATH = os.path.join(_HERE, "libgabe_host.so")
 
 class HostInit(C.Structure):
     _fields_ = [("f0", C.c_double * 2), ("df0", C.c_double * 2), ("rescale_B", C.c_double),
-                ("randseed", C.c_uint), ("rand_init", C.c_int)]
+                ("randseed", C.c_uint), ("rand_init", C.c_int), ("symmetric_modes", C.c_int),
+                ("spec_cut_off", C.c_double), ("spec_smooth", C.c_double)]
 
 
 _lib = None
@@ -41,6 +42,9 @@ def initialize(lat, cfg, rand_init=1):
     init.rescale_B = cfg.rescale_B
     init.randseed = cfg.randseed
     init.rand_init = rand_init
+    init.symmetric_modes = 0 if cfg.field_full_rand else 1
+    init.spec_cut_off = cfg.spec_cut_off
+    init.spec_smooth = cfg.spec_smooth
     _lat._check(load().gabe_host_initialize(lat._h, C.byref(lat.params), C.byref(init)))
 
 

@@ -60,6 +60,15 @@ const double endtime = (GABE_STEPS - 1) * dt;      /* the loop runs floor(endtim
 const int expansion_type = 1;
 
 const int rand_init = 1;
+#ifndef GABE_FIELD_FULL_RAND
+#define GABE_FIELD_FULL_RAND 1
+#endif
+#ifndef GABE_SPEC_CUT_OFF
+#define GABE_SPEC_CUT_OFF 0.     /* the reference leaves spec_cut_off undefined: no window */
+#endif
+const int field_full_rand = GABE_FIELD_FULL_RAND;   /* 0: symmetric k-space modes, needs spec_cut_off */
+const double spec_cut_off = GABE_SPEC_CUT_OFF;      /* window centre in units of N, e.g. sqrt(3.)/8. */
+const double spec_smooth = 0.5;                     /* sharpness of the tanh window */
 
 /* output parameters */
 const double screentime = 60;
@@ -73,7 +82,10 @@ const int slicewait = GABE_SLICEWAIT;
 const int slicenumber = GABE_SLICENUMBER;
 const int field_sliceskip = 2;
 const int specnumber = 1;
-const int field_outdim = 2;
+#ifndef GABE_FIELD_OUTDIM
+#define GABE_FIELD_OUTDIM 2
+#endif
+const int field_outdim = GABE_FIELD_OUTDIM;         /* 1: a line, 2: the plane i = 0, 3: the lattice */
 const int spec_output = 1;
 const int var_output = 1;
 

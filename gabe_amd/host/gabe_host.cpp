@@ -8,12 +8,14 @@
 #include <cstdlib>
 #include <vector>
 
-extern "C" int gabe_host_modes_from_draws(int N, double L, double rescale_B, double d2vdf2, double H0,
-                                          const int *draws, double *fk, double *fkd)
+extern "C" int gabe_host_modes_opt(int N, double L, double rescale_B, double d2vdf2, double H0,
+                                   const gabe_host_modes *opt, const int *draws, double *fk, double *fkd)
 {
-    if (N < 4 || !draws || !fk || !fkd) return GABE_ERR_ARG;
+    if (N < 4 || !opt || !fk || !fkd || (opt->full_rand && !draws)) return GABE_ERR_ARG;
+    if (!opt->full_rand && !(opt->cut_off > 0.)) return GABE_ERR_ARG;   /* field_full_rand 0 needs spec_cut_off */
     const int H = N / 2 + 1;
     const double dk2 = 4. * M_PI * M_PI / L / L;
+    const double s2 = std::sqrt(2.);
     long q = 0;
     for (int i = 0; i < N; i++) {
         const double i2 = (i < (N / 2 + 1) ? (i * i) : ((N - i) * (N - i)));
@@ -28,15 +30,27 @@ extern "C" int gabe_host_modes_from_draws(int N, double L, double rescale_B, dou
                     continue;
                 }
                 const double wk = std::sqrt(dk2 * (i2 + j2 + k2) + d2vdf2);
+                /* tanh window of g2init.cpp:122-129,158 */
+                const double win = opt->cut_off > 0.
+                                       ? (1. - std::tanh(opt->smooth * (std::sqrt(i2 + j2 + k2) - N * opt->cut_off))) / 2.
+                                       : 1.;
+                /* products in the reference's evaluation order, so the modes
+                 * agree to the last bit */
+                if (!opt->full_rand) {         /* g2init.cpp:153-165 */
+                    fk[2 * m] = rescale_B * std::sqrt(1 / (2. * wk * L * L * L)) * win;
+                    fk[2 * m + 1] = 0.;
+                    fkd[2 * m] = -H0 * fk[2 * m];
+                    fkd[2 * m + 1] = 0.;
+                    continue;
+                }
                 double r1[2], r2[2];
                 for (double *r : {r1, r2}) {   /* two (modulus, phase) draws per mode */
                     const double u = (double)draws[q++] / (double)RAND_MAX;
                     const double th = 2. * M_PI * (double)draws[q++] / (double)RAND_MAX;
                     const double mag = std::sqrt(-std::log(u) / (2. * wk * L * L * L));
-                    r[0] = rescale_B * std::cos(th) * mag;
-                    r[1] = rescale_B * std::sin(th) * mag;
+                    r[0] = rescale_B * std::cos(th) * mag * win;
+                    r[1] = rescale_B * std::sin(th) * mag * win;
                 }
-                const double s2 = std::sqrt(2.);
                 fk[2 * m] = (r1[0] + r2[0]) / s2;
                 fk[2 * m + 1] = (r1[1] + r2[1]) / s2;
                 fkd[2 * m] = (r2[1] - r1[1]) * wk / s2 - H0 * fk[2 * m];
@@ -45,6 +59,13 @@ extern "C" int gabe_host_modes_from_draws(int N, double L, double rescale_B, dou
         }
     }
     return GABE_OK;
+}
+
+extern "C" int gabe_host_modes_from_draws(int N, double L, double rescale_B, double d2vdf2, double H0,
+                                          const int *draws, double *fk, double *fkd)
+{
+    const gabe_host_modes opt{1, 0., 0.};
+    return gabe_host_modes_opt(N, L, rescale_B, d2vdf2, H0, &opt, draws, fk, fkd);
 }
 
 extern "C" long gabe_host_draws_per_field(int N) { return 4L * ((long)N * N * (N / 2 + 1) - 1); }
@@ -89,7 +110,9 @@ extern "C" int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const g
         /* every rand() value of every field is drawn first, in the reference's
          * order (field by field, g2model.cpp:131-133): device and FFT library
          * calls made in between must not be able to shift the sequence */
-        const long per = gabe_host_draws_per_field(N);
+        const gabe_host_modes opt{init->symmetric_modes ? 0 : 1, init->spec_cut_off, init->spec_smooth};
+        if (!opt.full_rand && !(opt.cut_off > 0.)) return GABE_ERR_ARG;
+        const long per = opt.full_rand ? gabe_host_draws_per_field(N) : 0;
         std::vector<int> draws((size_t)per * nf);
         srand(init->randseed);
         for (auto &d : draws) d = rand();
@@ -97,8 +120,8 @@ extern "C" int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const g
         for (int fld = 0; fld < nf; fld++) {
             double m2;
             if ((rc = gabe_effmass(ctx, fld, &m2)) != GABE_OK) return rc;
-            if ((rc = gabe_host_modes_from_draws(N, p->L, init->rescale_B, m2, H0, draws.data() + (size_t)fld * per,
-                                                 fk.data(), fkd.data())) != GABE_OK)
+            if ((rc = gabe_host_modes_opt(N, p->L, init->rescale_B, m2, H0, &opt, draws.data() + (size_t)fld * per,
+                                          fk.data(), fkd.data())) != GABE_OK)
                 return rc;
             if ((rc = gabe_add_fluctuation(ctx, fld, fk.data(), fkd.data())) != GABE_OK) return rc;
         }

@@ -17,7 +17,21 @@ typedef struct gabe_host_init {
     double rescale_B;           /* IC amplitude scale            (g2parameters.h:69) */
     unsigned randseed;          /* srand seed                    (g2parameters.h:56) */
     int rand_init;              /* 1: add vacuum fluctuations    (g2parameters.h:71) */
+    /* Appended options; zero keeps the reference defaults. */
+    int symmetric_modes;        /* 1: field_full_rand 0 (g2parameters.h:72), deterministic real
+                                 * modes with no rand() draws; needs spec_cut_off */
+    double spec_cut_off;        /* tanh window centre in units of N (g2parameters.h:73);
+                                 * <= 0 leaves it undefined (no window) */
+    double spec_smooth;         /* window sharpness              (g2parameters.h:74) */
 } gabe_host_init;
+
+/* Mode options of g2init.cpp:111-165: full_rand 1 draws, full_rand 0 the
+ * deterministic real modes; cut_off > 0 applies the tanh window. */
+typedef struct gabe_host_modes {
+    int full_rand;
+    double cut_off;
+    double smooth;
+} gabe_host_modes;
 
 /* Draw the half-spectrum vacuum modes of one field with libc rand() in the
  * sequential loop order of g2init.cpp:88-172 (field_full_rand 1).  fk and fkd
@@ -30,6 +44,11 @@ long gabe_host_draws_per_field(int N);
 /* The same modes from pre-drawn rand() values (draws[gabe_host_draws_per_field(N)]). */
 int gabe_host_modes_from_draws(int N, double L, double rescale_B, double d2vdf2, double H0, const int *draws,
                                double *fk, double *fkd);
+
+/* The modes under the options of g2init.cpp:111-165 (draws may be NULL when
+ * opt->full_rand is 0, which consumes none). */
+int gabe_host_modes_opt(int N, double L, double rescale_B, double d2vdf2, double H0, const gabe_host_modes *opt,
+                        const int *draws, double *fk, double *fkd);
 
 /* The start-up sequence of g2main.cpp:109-119: homogeneous fields and
  * calcEnergy(0); initexpansion(); srand(randseed) and per field (in order)

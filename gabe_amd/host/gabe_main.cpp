@@ -8,6 +8,7 @@
  * GPU: step() is gabe_step(), calcEnergy/adf come back through
  * gabe_get_scalars(), the spectra through gabe_spectra().
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,7 +29,12 @@ gabe_scalars sc{};
 double t = starttime;
 time_t tStart, tCurrent, tLast, tFinish;
 FILE *info = nullptr;
-std::vector<double> field0, dfield0;   /* host copy of slot 0, fetched at output slices */
+/* Host copy of the slot-0 planes the writers read, fetched at output slices:
+ * the whole lattice for field_outdim 3, otherwise the planes i = 0, +-1, +-2
+ * that the plane i = 0 and its 4th-order gradients need. */
+std::vector<double> field0, dfield0;
+std::vector<int> plane_slot;   /* lattice plane -> slot in field0/dfield0 (-1: not held) */
+long held_sites = 0;           /* sites per field held on the host */
 
 void die(int rc, const char *where)
 {
@@ -43,7 +49,7 @@ void die(int rc, const char *where)
 
 const gabe_model model{model_m2_phi, model_lambda, model_g2, model_m2_chi};
 
-inline long sidx(int i, int j, int k) { return ((long)i * N + j) * N + k; }
+inline long sidx(int i, int j, int k) { return ((long)plane_slot[i] * N + j) * N + k; }
 inline int incr(int i) { return i == N - 1 ? 0 : i + 1; }
 inline int decr(int i) { return i == 0 ? N - 1 : i - 1; }
 
@@ -51,7 +57,7 @@ inline int decr(int i) { return i == 0 ? N - 1 : i - 1; }
 double potential(int i, int j, int k)
 {
     const long n = sidx(i, j, k);
-    return gabe_V(model, field0[n], nflds > 1 ? field0[(long)N * N * N + n] : 0.);
+    return gabe_V(model, field0[n], nflds > 1 ? field0[held_sites + n] : 0.);
 }
 
 /* 4th-order first derivatives and |grad f|^2 (g2functions.cpp:70-122) */
@@ -77,10 +83,10 @@ double gradF2(const double *f, int i, int j, int k)
 double calcrho(int i, int j, int k)
 {
     double rho = potential(i, j, k);
-    const long N3 = (long)N * N * N, n = sidx(i, j, k);
+    const long n = sidx(i, j, k);
     for (int fld = 0; fld < nflds; fld++)
-        rho += (.5 * dfield0[fld * N3 + n] * dfield0[fld * N3 + n] +
-                .5 * gradF2(field0.data() + fld * N3, i, j, k) / sc.a[0] / sc.a[0]);
+        rho += (.5 * dfield0[fld * held_sites + n] * dfield0[fld * held_sites + n] +
+                .5 * gradF2(field0.data() + fld * held_sites, i, j, k) / sc.a[0] / sc.a[0]);
     return rho;
 }
 
@@ -156,25 +162,54 @@ void output_parameters()
 
 void fetch_state()
 {
-    const long n = (long)nflds * N * N * N;
-    field0.resize(n);
-    dfield0.resize(n);
-    CALL(gabe_download_state(ctx, field0.data(), dfield0.data()));
+    const long N2 = (long)N * N;
+    plane_slot.assign(N, -1);
+    if (field_outdim > 2) {
+        for (int i = 0; i < N; i++) plane_slot[i] = i;
+        held_sites = N2 * N;
+        field0.resize((size_t)nflds * held_sites);
+        dfield0.resize((size_t)nflds * held_sites);
+        CALL(gabe_download_state(ctx, field0.data(), dfield0.data()));
+        return;
+    }
+    std::vector<int> planes;
+    for (int d = -2; d <= 2; d++) {
+        const int i = ((d % N) + N) % N;
+        if (plane_slot[i] < 0) {
+            plane_slot[i] = (int)planes.size();
+            planes.push_back(i);
+        }
+    }
+    held_sites = (long)planes.size() * N2;
+    field0.resize((size_t)nflds * held_sites);
+    dfield0.resize((size_t)nflds * held_sites);
+    std::vector<double> pf((size_t)nflds * N2), pd((size_t)nflds * N2);
+    for (int i : planes) {
+        CALL(gabe_download_plane(ctx, i, pf.data(), pd.data()));
+        for (int fld = 0; fld < nflds; fld++) {
+            std::copy(pf.begin() + fld * N2, pf.begin() + (fld + 1) * N2,
+                      field0.begin() + fld * held_sites + plane_slot[i] * N2);
+            std::copy(pd.begin() + fld * N2, pd.begin() + (fld + 1) * N2,
+                      dfield0.begin() + fld * held_sites + plane_slot[i] * N2);
+        }
+    }
 }
 
-/* g2output.cpp:25-68 (field_outdim 2: the plane i = 0) */
+/* g2output.cpp:25-68: field_outdim 3 writes every field_sliceskip-th site of
+ * the lattice, 2 the plane i = 0, 1 the line i = j = 0 */
 void outputfield(int first)
 {
     char name[64];
     std::snprintf(name, sizeof name, "./slices/slices_fields_%d.dat", first);
     FILE *f = std::fopen(name, "w");
-    const int i = 0;
-    for (int j = 0; j < N; j += field_sliceskip)
-        for (int k = 0; k < N; k += field_sliceskip) {
-            for (int fld = 0; fld < nflds; fld++) std::fprintf(f, "%e ", field0[(long)fld * N * N * N + sidx(i, j, k)]);
-            std::fprintf(f, "%e ", calcrho(i, j, k));
-            std::fprintf(f, "\n");
-        }
+    const int ni = field_outdim > 2 ? N : 1, nj = field_outdim > 1 ? N : 1;
+    for (int i = 0; i < ni; i += field_sliceskip)
+        for (int j = 0; j < nj; j += field_sliceskip)
+            for (int k = 0; k < N; k += field_sliceskip) {
+                for (int fld = 0; fld < nflds; fld++) std::fprintf(f, "%e ", field0[fld * held_sites + sidx(i, j, k)]);
+                std::fprintf(f, "%e ", calcrho(i, j, k));
+                std::fprintf(f, "\n");
+            }
     std::fclose(f);
 }
 
@@ -200,15 +235,10 @@ void meansvars()
 {
     FILE *f = std::fopen("./slices/meansvariances.dat", "a");
     std::fprintf(f, "%f", t);
-    const long N3 = (long)N * N * N;
+    std::vector<double> mv(2 * nflds);
+    CALL(gabe_means_vars(ctx, mv.data()));   /* reduced on the device */
     for (int fld = 0; fld < nflds; fld++) {
-        double av = 0., var = 0.;
-        const double *x = field0.data() + fld * N3;
-        for (long n = 0; n < N3; n++) av += x[n];
-        av = av / gridsize;
-        const double av_sq = av * av;
-        for (long n = 0; n < N3; n++) var += x[n] * x[n] - av_sq;
-        var = var / gridsize;
+        const double av = mv[2 * fld], var = mv[2 * fld + 1];
         std::fprintf(f, " %e %e", av, var);
         if ((av != 0. && av / av != 1.)) {
             std::printf("Unstable solution developed. Field %d not numerical at t=%e\n", fld, t);
@@ -302,6 +332,9 @@ int main()
             init.rescale_B = rescale_B;
             init.randseed = (unsigned)randseed;
             init.rand_init = rand_init;
+            init.symmetric_modes = field_full_rand ? 0 : 1;
+            init.spec_cut_off = spec_cut_off;
+            init.spec_smooth = spec_smooth;
             CALL(gabe_host_initialize(ctx, &params, &init));
             std::printf("Fields initialized and fluctuated\n");
             screenout();

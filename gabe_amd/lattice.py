@@ -26,7 +26,8 @@ KERNEL_CLASSES = ("sweep_a", "finalize", "stress_fft", "gw_modes", "sweep_b")
 
 EXPORTS = (
     "gabe_create", "gabe_destroy", "gabe_last_error", "gabe_numbins",
-    "gabe_upload_state", "gabe_download_state", "gabe_upload_gw", "gabe_download_gw",
+    "gabe_upload_state", "gabe_download_state", "gabe_download_plane", "gabe_means_vars",
+    "gabe_upload_gw", "gabe_download_gw",
     "gabe_set_scalars", "gabe_get_scalars", "gabe_energy", "gabe_step", "gabe_sync",
     "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
     "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep",
@@ -76,6 +77,8 @@ def load():
         "gabe_numbins": [ip],
         "gabe_upload_state": [vp, dp, dp],
         "gabe_download_state": [vp, dp, dp],
+        "gabe_download_plane": [vp, ip, dp, dp],
+        "gabe_means_vars": [vp, dp],
         "gabe_upload_gw": [vp, dp, dp],
         "gabe_download_gw": [vp, dp, dp],
         "gabe_set_scalars": [vp, C.POINTER(GabeScalars)],
@@ -190,6 +193,20 @@ class Lattice:
         d = np.empty(shp)
         _check(load().gabe_download_state(self._h, _dp(f), _dp(d)))
         return f, d
+
+    def download_plane(self, i):
+        """Plane i of the state: (field, dfield), each [nflds][N][N]."""
+        shp = (self.nflds, self.N, self.N)
+        f = np.empty(shp)
+        d = np.empty(shp)
+        _check(load().gabe_download_plane(self._h, int(i), _dp(f), _dp(d)))
+        return f, d
+
+    def meansvars(self):
+        """[<f0>, var f0, <f1>, var f1] as in g2output.cpp:70-121."""
+        out = np.empty(2 * self.nflds)
+        _check(load().gabe_means_vars(self._h, _dp(out)))
+        return out
 
     def upload_gw(self, h=None, l=None):
         shp = (12, self.N, self.N, self.N // 2 + 1)

@@ -89,6 +89,17 @@ int gabe_numbins(int N);
 int gabe_upload_state(gabe_ctx *ctx, const double *field, const double *dfield);
 int gabe_download_state(gabe_ctx *ctx, double *field, double *dfield);
 
+/* Plane i of the slot-0 state, layout [nflds][N][N] each (either pointer may
+ * be NULL): what the slice writer of g2output.cpp:25-68 reads for
+ * field_outdim 1 and 2, and the i+-1, i+-2 planes the 4th-order gradients of
+ * calcrho (g2functions.cpp:70-122,176-186) need.  0 <= i < N. */
+int gabe_download_plane(gabe_ctx *ctx, int i, double *field, double *dfield);
+
+/* Mean and variance of each slot-0 field, out[2*fld] = <f>, out[2*fld+1] =
+ * <f^2> - <f>^2 (g2output.cpp:70-121), reduced on the device in FP64 with the
+ * first site's value as the shift of a one-pass sum. */
+int gabe_means_vars(gabe_ctx *ctx, double *out);
+
 /* Slot-0 tensor perturbations h and l = dh/dt, layout [12][N][N][N/2+1]
  * (g2main.cpp:30-35).  Either pointer may be NULL.  The device keeps each mode
  * as its two transverse-traceless polarisation amplitudes: the reference's

@@ -644,10 +644,17 @@ void orc_initexpansion(orc_state *st) /* g2init.cpp:16-43 */
     }
 }
 
-/* g2init.cpp:68-189 (field_full_rand 1, no spectral cut-off).  Draws rand()
- * in sequential loop order.  fk_out/fkd_out (optional, [N][N][N/2+1][2]) receive
- * the mode arrays before the inverse transform. */
-void orc_randinit(orc_state *st, int fld, double d2vdf2, double *fk_out, double *fkd_out)
+/* g2init.cpp:68-189.  full_rand 1 (field_full_rand 1, g2parameters.h:72):
+ * two (modulus, phase) rand() draws per mode in sequential loop order
+ * (g2init.cpp:111-150); full_rand 0: the deterministic real modes of
+ * g2init.cpp:153-165, no draws.  cut_off > 0 defines spec_cut_off
+ * (g2parameters.h:73): every amplitude is multiplied by the tanh window
+ * (1 - tanh(smooth (|n| - N cut_off)))/2 of g2init.cpp:117-129,158; full_rand 0
+ * requires it, as the reference does not compile without it.  fk_out/fkd_out
+ * (optional, [N][N][N/2+1][2]) receive the mode arrays before the inverse
+ * transform. */
+void orc_randinit_opt(orc_state *st, int fld, double d2vdf2, int full_rand, double cut_off, double smooth,
+                      double *fk_out, double *fkd_out)
 {
     int N = NN, H = N / 2 + 1;
     double L = st->p.L, B = st->p.rescale_B;
@@ -668,15 +675,23 @@ void orc_randinit(orc_state *st, int fld, double d2vdf2, double *fk_out, double 
                     continue;
                 }
                 double wk = sqrt(dk2 * ((double)i2 + (double)j2 + (double)k2) + d2vdf2);
+                double win = cut_off > 0. ? (1. - tanh(smooth * (sqrt(i2 + j2 + k2) - N * cut_off))) / 2. : 1.;
+                if (!full_rand) {
+                    fkf[2 * m] = B * sqrt(1 / (2. * wk * L * L * L)) * win;
+                    fkf[2 * m + 1] = 0.;
+                    fkd[2 * m] = -H0 * fkf[2 * m];
+                    fkd[2 * m + 1] = 0.;
+                    continue;
+                }
                 double RI1[2], RI2[2];
                 double randVar = (double)rand() / (double)RAND_MAX;
                 double randAngle = 2. * M_PI * (double)rand() / (double)RAND_MAX;
-                RI1[0] = B * cos(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L));
-                RI1[1] = B * sin(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L));
+                RI1[0] = B * cos(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L)) * win;
+                RI1[1] = B * sin(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L)) * win;
                 randVar = (double)rand() / (double)RAND_MAX;
                 randAngle = 2. * M_PI * (double)rand() / (double)RAND_MAX;
-                RI2[0] = B * cos(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L));
-                RI2[1] = B * sin(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L));
+                RI2[0] = B * cos(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L)) * win;
+                RI2[1] = B * sin(randAngle) * sqrt(-log(randVar) / (2. * wk * L * L * L)) * win;
                 fkf[2 * m] = (RI1[0] + RI2[0]) / sqrt(2.);
                 fkf[2 * m + 1] = (RI1[1] + RI2[1]) / sqrt(2.);
                 fkd[2 * m] = (RI2[1] - RI1[1]) * wk / sqrt(2.) - H0 * fkf[2 * m];
@@ -694,6 +709,12 @@ void orc_randinit(orc_state *st, int fld, double d2vdf2, double *fk_out, double 
     free(inic);
     free(fkf);
     free(fkd);
+}
+
+/* the reference default: field_full_rand 1, spec_cut_off not defined */
+void orc_randinit(orc_state *st, int fld, double d2vdf2, double *fk_out, double *fkd_out)
+{
+    orc_randinit_opt(st, fld, d2vdf2, 1, 0., 0., fk_out, fkd_out);
 }
 
 void orc_initfields_fluct(orc_state *st) /* g2model.cpp:132-145 (seeded by the caller) */

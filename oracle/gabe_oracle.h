@@ -87,6 +87,8 @@ int orc_rand(void);
 void orc_initfields_homog(orc_state *st, const double *f0, const double *df0);
 void orc_initexpansion(orc_state *st);
 void orc_randinit(orc_state *st, int fld, double d2vdf2, double *fk_out, double *fkd_out);
+void orc_randinit_opt(orc_state *st, int fld, double d2vdf2, int full_rand, double cut_off, double smooth,
+                      double *fk_out, double *fkd_out);
 void orc_initfields_fluct(orc_state *st);
 
 /* diagnostics (g2output.cpp) */

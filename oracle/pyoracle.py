@@ -71,6 +71,7 @@ def lib():
             "orc_initfields_homog": (None, [P, dp, dp]),
             "orc_initexpansion": (None, [P]),
             "orc_randinit": (None, [P, C.c_int, C.c_double, dp, dp]),
+            "orc_randinit_opt": (None, [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, dp, dp]),
             "orc_initfields_fluct": (None, [P]),
             "orc_meansvars": (None, [P, dp]),
             "orc_r2c3d": (None, [C.c_int, dp, dp, C.c_int]),
@@ -210,13 +211,15 @@ class Oracle:
     def initexpansion(self):
         lib().orc_initexpansion(self._st)
 
-    def randinit(self, fld, d2vdf2, want_modes=False):
+    def randinit(self, fld, d2vdf2, want_modes=False, full_rand=1, cut_off=0.0, smooth=0.0):
+        """g2init.cpp:68-189; cut_off <= 0 leaves spec_cut_off undefined."""
         if want_modes:
             fk = np.zeros((self.N, self.N, self.N // 2 + 1), dtype=np.complex128)
             fkd = np.zeros_like(fk)
-            lib().orc_randinit(self._st, fld, d2vdf2, _dp(fk.view(np.float64)), _dp(fkd.view(np.float64)))
+            lib().orc_randinit_opt(self._st, fld, d2vdf2, full_rand, cut_off, smooth,
+                                   _dp(fk.view(np.float64)), _dp(fkd.view(np.float64)))
             return fk, fkd
-        lib().orc_randinit(self._st, fld, d2vdf2, None, None)
+        lib().orc_randinit_opt(self._st, fld, d2vdf2, full_rand, cut_off, smooth, None, None)
         return None
 
     def initfields_fluct(self):

@@ -24,7 +24,13 @@ def seeded_oracle(cfg, amp=None, nthreads=4):
     o.initfields_homog(cfg.f0, cfg.df0)
     o.initexpansion()
     po.srand(cfg.randseed)
-    o.initfields_fluct()
+    if cfg.field_full_rand and cfg.spec_cut_off <= 0:
+        o.initfields_fluct()
+    else:   # g2model.cpp:132-145 with the options of g2init.cpp:111-165
+        for fld in range(cfg.nflds):
+            o.randinit(fld, o.effMass(0, fld), full_rand=cfg.field_full_rand, cut_off=cfg.spec_cut_off,
+                       smooth=cfg.spec_smooth)
+        o.calcEnergy(0)
     return o
 
 

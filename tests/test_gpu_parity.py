@@ -8,6 +8,8 @@ Tolerances:
   GW modes within 1e-3 of their amplitude, a/adot within 1e-10 relative,
   over the early window tested here.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -135,13 +137,17 @@ def test_effmass_and_fluctuation_add_match_oracle():
     assert fluct_err(d, o.dfield[0]) < 1e-11
 
 
-@pytest.mark.parametrize("precision", [GABE_FP64, GABE_FP32])
-def test_host_initialize_matches_reference_init(precision):
+@pytest.mark.parametrize("precision,full_rand,cut", [(GABE_FP64, 1, 0.0), (GABE_FP32, 1, 0.0),
+                                                    (GABE_FP64, 1, math.sqrt(3) / 8), (GABE_FP64, 0, math.sqrt(3) / 8)])
+def test_host_initialize_matches_reference_init(precision, full_rand, cut):
     """gabe_host_initialize (the kept host's start-up) vs the oracle's
-    g2main.cpp:109-119 sequence: same fields, energies and adot[0]."""
+    g2main.cpp:109-119 sequence: same fields, energies and adot[0], for the
+    default modes, the windowed modes and the symmetric modes."""
     from gabe_amd import host
     cfg = configs.lambda_phi4_config(N=24, calc_gws=1)
     cfg.rescale_B = 1e-3
+    cfg.field_full_rand = full_rand
+    cfg.spec_cut_off = cut
     o = seeded_oracle(cfg)
     lat = host.initialized_lattice(cfg, precision=precision)
     f, d = lat.download_state()
@@ -153,6 +159,35 @@ def test_host_initialize_matches_reference_init(precision):
     # FP32 stores the fluctuation velocity in FP32: rho carries ~1e-12 of its rounding
     etol = 1e-12 if precision == GABE_FP64 else 1e-10
     assert abs(got["edrho"][0] - ref["edrho"][0]) <= etol * ref["edrho"][0]
+
+
+@pytest.mark.parametrize("precision", [GABE_FP64, GABE_FP32])
+def test_planes_and_means_vars(precision):
+    """gabe_download_plane returns the planes of gabe_download_state;
+    gabe_means_vars matches a two-pass numpy reduction of the downloaded
+    state (mean to 1e-15 of the field, variance to 1e-9) and the oracle's
+    g2output.cpp:70-121 formula."""
+    cfg = configs.lambda_phi4_config(N=24, calc_gws=0)
+    o = seeded_oracle(cfg, amp=2e-3)
+    lat = lattice_from_oracle(o, cfg, precision)
+    lat.step(5)
+    f, d = lat.download_state()
+    for i in (0, 1, 23):
+        pf, pd = lat.download_plane(i)
+        assert np.array_equal(pf, f[:, i]) and np.array_equal(pd, d[:, i])
+    with pytest.raises(Exception):
+        lat.download_plane(24)
+    mv = lat.meansvars()
+    o.step(5)
+    ref = o.meansvars()
+    for fld in range(2):
+        x = f[fld].astype(np.float64)
+        m = x.mean()
+        v = ((x - m) ** 2).mean()
+        assert abs(mv[2 * fld] - m) <= 1e-15 * max(abs(m), 1e-3)
+        assert abs(mv[2 * fld + 1] - v) <= 1e-9 * v
+        tol = 1e-9 if precision == GABE_FP64 else 1e-5
+        assert abs(mv[2 * fld + 1] - ref[2 * fld + 1]) <= tol * ref[2 * fld + 1]
 
 
 @pytest.mark.parametrize("N", [32, 48])
