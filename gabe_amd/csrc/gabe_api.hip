@@ -637,6 +637,132 @@ int gabe_means_vars(gabe_ctx *c, double *out)
     return GABE_OK;
 }
 
+/* Checkpoint blob: a header, then the device scalars, the history rows in
+ * use (oldest first), the current slot's field and velocity arrays and, with
+ * tensor modes, h and l, all in the device's own storage precision, so a
+ * restored context continues bit for bit. */
+struct CkptHeader {
+    char magic[8];
+    unsigned version;
+    int N, nflds, precision, calc_gws, expansion_type;
+    double L, dt, m2_phi, lambda, g2, m2_chi;
+    int gw_spectrum_quirk, hist_rows;
+    unsigned long long state_bytes, gw_bytes, total;
+};
+static const char kCkptMagic[8] = {'G', 'A', 'B', 'E', 'S', 'T', 'A', 'T'};
+
+static CkptHeader ckpt_header(const gabe_ctx *c, int hist_rows)
+{
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.N = c->N;
+    h.nflds = c->nf;
+    h.precision = c->p.precision;
+    h.calc_gws = c->p.calc_gws;
+    h.expansion_type = c->p.expansion_type;
+    h.L = c->p.L;
+    h.dt = c->p.dt;
+    h.m2_phi = c->p.m2_phi;
+    h.lambda = c->p.lambda;
+    h.g2 = c->p.g2;
+    h.m2_chi = c->p.m2_chi;
+    h.gw_spectrum_quirk = c->p.gw_spectrum_quirk;
+    h.hist_rows = hist_rows;
+    h.state_bytes = (unsigned long long)c->nf * c->N3 * c->esz;
+    h.gw_bytes = c->alloc_gw ? 2ULL * c->esz * 2 * c->M : 0;
+    h.total = sizeof(CkptHeader) + sizeof(DevScalars) + sizeof(double) * GABE_HIST_COLS * (unsigned long long)hist_rows +
+              2 * h.state_bytes + 2 * h.gw_bytes;
+    return h;
+}
+
+int gabe_checkpoint_size(gabe_ctx *c, size_t *bytes)
+{
+    if (!c || !bytes) return fail(GABE_ERR_ARG, "NULL argument");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    *bytes = ckpt_header(c, (int)std::min<long long>(d.nstep, c->hist_cap)).total;
+    return GABE_OK;
+}
+
+int gabe_checkpoint_save(gabe_ctx *c, void *buf, size_t bytes)
+{
+    if (!c || !buf) return fail(GABE_ERR_ARG, "NULL argument");
+    DevScalars d;
+    CHK(read_scalars(c, d));
+    const int rows = (int)std::min<long long>(d.nstep, c->hist_cap);
+    const CkptHeader h = ckpt_header(c, rows);
+    if (bytes < h.total) return fail(GABE_ERR_ARG, "checkpoint buffer holds %zu bytes, %llu needed", bytes, h.total);
+    char *p = (char *)buf;
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    std::memcpy(p, &d, sizeof d);
+    p += sizeof d;
+    if (rows) {
+        std::vector<double> ring((size_t)c->hist_cap * GABE_HIST_COLS);
+        HIPCHK(hipMemcpy(ring.data(), c->hist, sizeof(double) * ring.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < rows; r++) {
+            const long stepno = (long)d.nstep - rows + r;
+            std::memcpy(p, ring.data() + (stepno % c->hist_cap) * GABE_HIST_COLS, sizeof(double) * GABE_HIST_COLS);
+            p += sizeof(double) * GABE_HIST_COLS;
+        }
+    }
+    for (const void *src : {c->fbuf[c->cur], c->vbuf[c->cur]}) {
+        HIPCHK(hipMemcpy(p, src, h.state_bytes, hipMemcpyDeviceToHost));
+        p += h.state_bytes;
+    }
+    if (c->alloc_gw)
+        for (const void *src : {(const void *)c->h, (const void *)c->l}) {
+            HIPCHK(hipMemcpy(p, src, h.gw_bytes, hipMemcpyDeviceToHost));
+            p += h.gw_bytes;
+        }
+    return GABE_OK;
+}
+
+int gabe_checkpoint_load(gabe_ctx *c, const void *buf, size_t bytes)
+{
+    if (!c || !buf) return fail(GABE_ERR_ARG, "NULL argument");
+    if (bytes < sizeof(CkptHeader)) return fail(GABE_ERR_ARG, "checkpoint shorter than its header");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) return fail(GABE_ERR_ARG, "not a gabe checkpoint");
+    const CkptHeader mine = ckpt_header(c, h.hist_rows);
+    if (h.N != mine.N || h.nflds != mine.nflds || h.precision != mine.precision || h.calc_gws != mine.calc_gws ||
+        h.expansion_type != mine.expansion_type || h.L != mine.L || h.dt != mine.dt || h.m2_phi != mine.m2_phi ||
+        h.lambda != mine.lambda || h.g2 != mine.g2 || h.m2_chi != mine.m2_chi ||
+        h.gw_spectrum_quirk != mine.gw_spectrum_quirk)
+        return fail(GABE_ERR_ARG, "checkpoint was written by a context with other parameters (N=%d nflds=%d "
+                                  "precision=%d calc_gws=%d)", h.N, h.nflds, h.precision, h.calc_gws);
+    if (h.hist_rows < 0 || h.hist_rows > c->hist_cap || h.total != mine.total || bytes < h.total)
+        return fail(GABE_ERR_ARG, "checkpoint size does not match its header");
+    const char *p = (const char *)buf + sizeof h;
+    DevScalars d;
+    std::memcpy(&d, p, sizeof d);
+    p += sizeof d;
+    if (d.nstep < h.hist_rows) return fail(GABE_ERR_ARG, "checkpoint history longer than its step count");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h.hist_rows) {
+        std::vector<double> ring((size_t)c->hist_cap * GABE_HIST_COLS, 0.);
+        for (int r = 0; r < h.hist_rows; r++) {
+            const long stepno = (long)d.nstep - h.hist_rows + r;
+            std::memcpy(ring.data() + (stepno % c->hist_cap) * GABE_HIST_COLS, p, sizeof(double) * GABE_HIST_COLS);
+            p += sizeof(double) * GABE_HIST_COLS;
+        }
+        HIPCHK(hipMemcpy(c->hist, ring.data(), sizeof(double) * ring.size(), hipMemcpyHostToDevice));
+    }
+    for (void *dst : {c->fbuf[c->cur], c->vbuf[c->cur]}) {
+        HIPCHK(hipMemcpy(dst, p, h.state_bytes, hipMemcpyHostToDevice));
+        p += h.state_bytes;
+    }
+    if (c->alloc_gw)
+        for (void *dst : {c->h, c->l}) {
+            HIPCHK(hipMemcpy(dst, p, h.gw_bytes, hipMemcpyHostToDevice));
+            p += h.gw_bytes;
+        }
+    CHK(write_scalars(c, d));
+    return GABE_OK;
+}
+
 int gabe_upload_gw(gabe_ctx *c, const double *h, const double *l)
 {
     if (!c) return fail(GABE_ERR_ARG, "NULL context");

@@ -89,6 +89,12 @@ const int field_outdim = GABE_FIELD_OUTDIM;         /* 1: a line, 2: the plane i
 const int spec_output = 1;
 const int var_output = 1;
 
+/* checkpoint.bin every this many steps (0: never); not in the reference */
+#ifndef GABE_CHECKPOINT_EVERY
+#define GABE_CHECKPOINT_EVERY 0
+#endif
+const int checkpoint_every = GABE_CHECKPOINT_EVERY;
+
 const int nflds = num_flds;
 const double dx = L / ((double)N);
 const double gridsize = (double)N * N * N;

@@ -6,6 +6,9 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <string>
+#include <cstdio>
 #include <vector>
 
 extern "C" int gabe_host_modes_opt(int N, double L, double rescale_B, double d2vdf2, double H0,
@@ -127,6 +130,55 @@ extern "C" int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const g
         }
     }
     return gabe_energy(ctx, 0);
+}
+
+namespace {
+const char kRunMagic[8] = {'G', 'A', 'B', 'E', 'R', 'U', 'N', '1'};
+}
+
+extern "C" int gabe_host_checkpoint_write(gabe_ctx *ctx, const char *path, const gabe_host_run *run)
+{
+    if (!ctx || !path || !run) return GABE_ERR_ARG;
+    size_t n = 0;
+    int rc = gabe_checkpoint_size(ctx, &n);
+    if (rc != GABE_OK) return rc;
+    std::vector<char> blob(n);
+    if ((rc = gabe_checkpoint_save(ctx, blob.data(), n)) != GABE_OK) return rc;
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return GABE_ERR_ARG;
+    const unsigned long long len = n;
+    bool ok = std::fwrite(kRunMagic, 1, 8, f) == 8 && std::fwrite(run, sizeof *run, 1, f) == 1 &&
+              std::fwrite(&len, sizeof len, 1, f) == 1 && std::fwrite(blob.data(), 1, n, f) == n;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return GABE_ERR_ARG;
+    }
+    return GABE_OK;
+}
+
+extern "C" int gabe_host_checkpoint_read(gabe_ctx *ctx, const char *path, gabe_host_run *run)
+{
+    if (!ctx || !path || !run) return GABE_ERR_ARG;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return GABE_ERR_ARG;
+    char magic[8];
+    gabe_host_run r;
+    unsigned long long len = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kRunMagic, 8) == 0 &&
+              std::fread(&r, sizeof r, 1, f) == 1 && std::fread(&len, sizeof len, 1, f) == 1 && len < (1ULL << 40);
+    std::vector<char> blob;
+    if (ok) {
+        blob.resize((size_t)len);
+        ok = std::fread(blob.data(), 1, (size_t)len, f) == (size_t)len;
+    }
+    std::fclose(f);
+    if (!ok) return GABE_ERR_ARG;
+    const int rc = gabe_checkpoint_load(ctx, blob.data(), (size_t)len);
+    if (rc != GABE_OK) return rc;
+    *run = r;
+    return GABE_OK;
 }
 
 extern "C" double gabe_host_friedmann_residual(const double *hist, int nrows, double *out)

@@ -56,6 +56,25 @@ int gabe_host_modes_opt(int N, double L, double rescale_B, double d2vdf2, double
  * homogeneous value, as in the reference. */
 int gabe_host_initialize(gabe_ctx *ctx, const gabe_params *p, const gabe_host_init *init);
 
+/* Where a run stands, saved beside the lattice in a checkpoint file: the
+ * loop time t (g2main.cpp:106), steps taken, the rand() seed and the number
+ * of values drawn since srand (all draws happen at start-up, g2init.cpp:84),
+ * and the slice writer's counters (g2output.cpp:129-186). */
+typedef struct gabe_host_run {
+    double t;
+    long long step;
+    unsigned randseed;
+    long long rand_draws;
+    int slice_first, slice_last, spec_last, slice_wait;
+} gabe_host_run;
+
+/* Write the run record and gabe_checkpoint_save's blob to path (through a
+ * temporary file renamed into place, so a crash leaves the previous file). */
+int gabe_host_checkpoint_write(gabe_ctx *ctx, const char *path, const gabe_host_run *run);
+
+/* Read a file written by gabe_host_checkpoint_write into ctx and *run. */
+int gabe_host_checkpoint_read(gabe_ctx *ctx, const char *path, gabe_host_run *run);
+
 /* Friedmann-constraint residual from a per-step scalar history (rows of
  * gabe_history: t, a, adot, edkin, edpot, edgrad, edrho, step).  For each
  * interior row n: R = |rho_dot + 3 H (rho + p)| / (3 H rho), with

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <ctime>
 #include <sys/stat.h>
 #include <vector>
@@ -254,10 +255,12 @@ void meansvars()
 int slicewaitf() { return (int)(endtime / dt / slicenumber); }
 
 /* g2output.cpp:129-186: decides whether this call writes a slice */
+/* the writer's counters live at file scope so a checkpoint can carry them */
+int slice_first = 0, slice_last = 0, spec_last = 0, slice_waitm = 0;
+
 void outputslice()
 {
-    static int first = 0;
-    static int lastslice, lastspec, slicewaitm;
+    int &first = slice_first, &lastslice = slice_last, &lastspec = spec_last, &slicewaitm = slice_waitm;
     if (first == 0) {
         slicewaitm = slicewait == 0 ? slicewaitf() : slicewait;
         lastslice = slicewaitm;
@@ -297,10 +300,38 @@ void screenout()
     }
 }
 
+/* Checkpoint of the run after `step` steps at loop time t (not in the
+ * reference, which has none). */
+void write_checkpoint(const char *path, long long step)
+{
+    gabe_host_run run{};
+    run.t = t;
+    run.step = step;
+    run.randseed = (unsigned)randseed;
+    run.rand_draws = (rand_init && field_full_rand) ? nflds * gabe_host_draws_per_field(N) : 0;
+    run.slice_first = slice_first;
+    run.slice_last = slice_last;
+    run.spec_last = spec_last;
+    run.slice_wait = slice_waitm;
+    CALL(gabe_host_checkpoint_write(ctx, path, &run));
+}
+
 } // namespace
 
-int main()
+int main(int argc, char **argv)
 {
+    const char *resume = nullptr;
+    long long stop_at = -1;
+    for (int a = 1; a < argc; a++) {
+        if (!std::strcmp(argv[a], "--resume") && a + 1 < argc) {
+            resume = argv[++a];
+        } else if (!std::strcmp(argv[a], "--checkpoint-at") && a + 1 < argc) {
+            stop_at = std::atoll(argv[++a]);
+        } else {
+            std::fprintf(stderr, "usage: %s [--resume FILE] [--checkpoint-at STEP]\n", argv[0]);
+            return 2;
+        }
+    }
     std::printf("\n\nGABE started!\n\n");
     output_parameters();
     std::printf("Info file made\n");
@@ -322,8 +353,23 @@ int main()
     std::printf("Memory allocated for all arrays\n\n");
     std::printf("Starting run...\n\n");
 
-    for (t = starttime; t <= endtime; t += dt) {   /* g2main.cpp:106-131 */
-        if (t == starttime) {
+    long long step = 0;
+    bool resumed = false;
+    if (resume) {
+        gabe_host_run run{};
+        CALL(gabe_host_checkpoint_read(ctx, resume, &run));
+        t = run.t;
+        step = run.step;
+        slice_first = run.slice_first;
+        slice_last = run.slice_last;
+        spec_last = run.spec_last;
+        slice_waitm = run.slice_wait;
+        resumed = true;
+        t += dt;   /* the loop's increment after the step the checkpoint followed */
+        std::printf("Resumed from %s at step %lld\n", resume, step);
+    }
+    for (t = resumed ? t : starttime; t <= endtime; t += dt) {   /* g2main.cpp:106-131 */
+        if (!resumed && t == starttime) {
             gabe_host_init init{};
             for (int fld = 0; fld < nflds; fld++) {
                 init.f0[fld] = f0[fld];
@@ -341,8 +387,16 @@ int main()
             outputslice();
         }
         CALL(gabe_step(ctx, 1));
+        step++;
         screenout();
         outputslice();
+        if (checkpoint_every > 0 && step % checkpoint_every == 0) write_checkpoint("checkpoint.bin", step);
+        if (step == stop_at) {
+            write_checkpoint("checkpoint.bin", step);
+            std::printf("Checkpoint written after step %lld\n", step);
+            gabe_destroy(ctx);
+            return 0;
+        }
     }
     CALL(gabe_sync(ctx));
     CALL(gabe_get_scalars(ctx, &sc));

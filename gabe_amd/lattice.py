@@ -27,7 +27,7 @@ KERNEL_CLASSES = ("sweep_a", "finalize", "stress_fft", "gw_modes", "sweep_b")
 EXPORTS = (
     "gabe_create", "gabe_destroy", "gabe_last_error", "gabe_numbins",
     "gabe_upload_state", "gabe_download_state", "gabe_download_plane", "gabe_means_vars",
-    "gabe_upload_gw", "gabe_download_gw",
+    "gabe_upload_gw", "gabe_download_gw", "gabe_checkpoint_size", "gabe_checkpoint_save", "gabe_checkpoint_load",
     "gabe_set_scalars", "gabe_get_scalars", "gabe_energy", "gabe_step", "gabe_sync",
     "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
     "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep",
@@ -79,6 +79,9 @@ def load():
         "gabe_download_state": [vp, dp, dp],
         "gabe_download_plane": [vp, ip, dp, dp],
         "gabe_means_vars": [vp, dp],
+        "gabe_checkpoint_size": [vp, C.POINTER(C.c_size_t)],
+        "gabe_checkpoint_save": [vp, vp, C.c_size_t],
+        "gabe_checkpoint_load": [vp, vp, C.c_size_t],
         "gabe_upload_gw": [vp, dp, dp],
         "gabe_download_gw": [vp, dp, dp],
         "gabe_set_scalars": [vp, C.POINTER(GabeScalars)],
@@ -207,6 +210,18 @@ class Lattice:
         out = np.empty(2 * self.nflds)
         _check(load().gabe_means_vars(self._h, _dp(out)))
         return out
+
+    def checkpoint(self):
+        """The context's state as a bytes-like uint8 array (gabe_checkpoint_save)."""
+        n = C.c_size_t()
+        _check(load().gabe_checkpoint_size(self._h, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        _check(load().gabe_checkpoint_save(self._h, buf.ctypes.data, n.value))
+        return buf
+
+    def restore(self, blob):
+        b = np.ascontiguousarray(np.frombuffer(blob, dtype=np.uint8))
+        _check(load().gabe_checkpoint_load(self._h, b.ctypes.data, b.size))
 
     def upload_gw(self, h=None, l=None):
         shp = (12, self.N, self.N, self.N // 2 + 1)

@@ -25,6 +25,8 @@
 #ifndef GABE_H
 #define GABE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -99,6 +101,18 @@ int gabe_download_plane(gabe_ctx *ctx, int i, double *field, double *dfield);
  * <f^2> - <f>^2 (g2output.cpp:70-121), reduced on the device in FP64 with the
  * first site's value as the shift of a one-pass sum. */
 int gabe_means_vars(gabe_ctx *ctx, double *out);
+
+/* Checkpoint and resume (no counterpart in the reference, which has none).
+ * The blob holds the parameters that fix the dynamics, the device scalars
+ * (a, adot, energies, FP64 means, t, step count), the history rows in use and
+ * the state (fields, velocities and, with tensor modes, h and l) in the
+ * device's storage precision, so a context restored from it continues bit
+ * for bit.  gabe_checkpoint_size gives the bytes gabe_checkpoint_save writes;
+ * gabe_checkpoint_load accepts only a blob from a context with the same N,
+ * nflds, precision, calc_gws, expansion_type, L, dt and model coefficients. */
+int gabe_checkpoint_size(gabe_ctx *ctx, size_t *bytes);
+int gabe_checkpoint_save(gabe_ctx *ctx, void *buf, size_t bytes);
+int gabe_checkpoint_load(gabe_ctx *ctx, const void *buf, size_t bytes);
 
 /* Slot-0 tensor perturbations h and l = dh/dt, layout [12][N][N][N/2+1]
  * (g2main.cpp:30-35).  Either pointer may be NULL.  The device keeps each mode
