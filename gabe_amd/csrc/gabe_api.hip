@@ -18,9 +18,10 @@
 #include "gabe.h"
 #include "gabe_fft.h"
 #include "gabe_gwfft.h"
+#include "gabe_ctx.h"
 #include "gabe_internal.h"
 
-namespace {
+namespace gabe_detail {
 
 thread_local std::string g_err;
 
@@ -35,26 +36,8 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(GABE_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                \
-    } while (0)
 
-#define FFTCHK(expr)                                                                        \
-    do {                                                                                    \
-        rocfft_status s_ = (expr);                                                          \
-        if (s_ != rocfft_status_success)                                                    \
-            return fail(GABE_ERR_FFT, "%s failed: rocfft status %d", #expr, (int)s_);      \
-    } while (0)
 
-#define CHK(expr)                        \
-    do {                                 \
-        int r_ = (expr);                 \
-        if (r_ != GABE_OK) return r_;    \
-    } while (0)
 
 bool g_rocfft_ready = false;
 
@@ -71,51 +54,12 @@ double csum(const double *x, long n)
     return s + c;
 }
 
-} // namespace
+} // namespace gabe_detail
 
-struct FftPlan {
-    rocfft_plan plan = nullptr;
-    rocfft_execution_info info = nullptr;
-    size_t work = 0;
-};
+using namespace gabe_detail;
 
-struct gabe_ctx {
-    gabe_params p{};
-    int N = 0, nf = 0;
-    long N2 = 0, N3 = 0, M = 0;
-    double dx = 0.;
-    bool fp32 = false, gw = false, alloc_gw = false;
-    size_t esz = 8;              /* bytes per stored real */
-    hipStream_t stream = nullptr;
-    bool own_stream = true;          /* false for ensemble members (the ensemble's stream) */
-    hipStream_t side = nullptr;      /* sweep B runs here, beside the stress transform and GW chain */
-    hipEvent_t ev_fin = nullptr, ev_side = nullptr;
-    void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
-    void *vbuf[2] = {nullptr, nullptr};
-    int cur = 0;
-    void *stress = nullptr;      /* [ns_arrays][N3] real */
-    void *shat = nullptr;        /* [ns_arrays][M] complex */
-    void *h = nullptr, *l = nullptr; /* [6][M] complex */
-    double *partials = nullptr;
-    int nblk = 0, ci = 0;
-    dim3 grid;
-    DevScalars *sc = nullptr;
-    StepScalars *ss = nullptr;
-    double *hist = nullptr;
-    int hist_cap = 0;
-    double *spec_out = nullptr;
-    FftPlan fft_stress, fft_spec, fft_c2r;
-    bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
-    bool gwfft = false;          /* i columns fused with the tensor-mode update */
-    long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
-    int fft_cw = 16;             /* column tile width of the j/i transforms */
-    void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */
-    void *fft_work = nullptr;
-    size_t fft_work_size = 0;
-    gabe_model model{};
-};
 
-namespace {
+namespace gabe_detail {
 
 int ensure_rocfft()
 {
@@ -126,8 +70,8 @@ int ensure_rocfft()
     return GABE_OK;
 }
 
-int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch, int rank = 3,
-              size_t out_distance = 0)
+int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch, int rank,
+              size_t out_distance)
 {
     CHK(ensure_rocfft());
     size_t len[3] = {(size_t)c->N, (size_t)c->N, (size_t)c->N};
@@ -316,6 +260,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     else CHK(launch_sweep_ctx(c, SW_B, false));
     CHK(rec(9));
     c->cur = 1 - c->cur;
+    if (c->closure) CHK(closure_after_step(c));
     return GABE_OK;
 }
 
@@ -470,11 +415,14 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
 
 void destroy_impl(gabe_ctx *c)
 {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->closure) closure_release(c);
     destroy_plan(c->fft_stress);
     destroy_plan(c->fft_spec);
     destroy_plan(c->fft_c2r);
+    destroy_plan(c->fft_rx);
     void *bufs[] = {c->fbuf[0], c->fbuf[1], c->vbuf[0], c->vbuf[1], c->stress, c->shat, c->h, c->l,
-                    c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work, c->twiddle};
+                    c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work, c->twiddle, c->rx_buf};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->side) (void)hipStreamSynchronize(c->side);
@@ -519,7 +467,7 @@ int download_fields(gabe_ctx *c, const void *src, double *dst, const double *mea
     return GABE_OK;
 }
 
-} // namespace
+} // namespace gabe_detail
 
 /* ====================================================================== ABI */
 extern "C" {

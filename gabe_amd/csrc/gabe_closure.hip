@@ -1,0 +1,1192 @@
+/*
+ * gabe_closure.hip -- learned closure of a coarse lattice and the spectral
+ * restriction that produces its training data.  No counterpart in the
+ * reference.
+ *
+ * Network (per site, periodic 3^3 stencils, channel-last activations):
+ *   x  = (dphi, dchi, dphidot, dchidot) / s_in            4 channels, BF16
+ *   a1 = SiLU(conv3(x;  W1[16][4][27])  + b1)            16 channels, BF16
+ *   a2 = SiLU(conv3(a1; W2[16][16][27]) + b2)            16 channels, BF16
+ *   y  =      conv3(a2; W3[2][16][27])  + b3              2 channels, FP32
+ * conv3(u; W)[s][co] = sum_t sum_ci W[co][ci][t] u[s + off(t)][ci], tap
+ * t = 9 (dz+1) + 3 (dy+1) + (dx+1), dz along i (slowest), dx along k.
+ * After every RK2 step the velocity fluctuations get
+ *   dfdot_c += dt s_out[c] y_c / a^2,
+ * the 1/a^2 carrying the metric factor of the Laplacian the closure corrects.
+ *
+ * Every convolution is an implicit GEMM on v_mfma_f32_16x16x32_bf16: one
+ * wave computes 16 consecutive sites (rows) x 16 output channels (columns);
+ * K runs over (tap, input channel): two taps of 16 channels, or eight taps of
+ * 4 channels, per instruction.  FP32 accumulation, BF16 operands.  The
+ * backward pass reuses the same kernel on the gradients with the weights
+ * flipped in space and transposed in channels; the weight gradient is a
+ * second implicit GEMM with K over sites, fed from LDS copies of the
+ * activations transposed to channel-major, reduced over workgroups in a fixed
+ * order.  Training is Adam on FP32 master weights, repacked to BF16 after
+ * every update.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gabe_ctx.h"
+
+using namespace gabe_detail;
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int CL_IN = 4, CL_H = 16, CL_OUT = 2, CL_TAPS = 27;
+constexpr int P_W1 = 0;
+constexpr int P_B1 = P_W1 + CL_H * CL_IN * CL_TAPS;
+constexpr int P_W2 = P_B1 + CL_H;
+constexpr int P_B2 = P_W2 + CL_H * CL_H * CL_TAPS;
+constexpr int P_W3 = P_B2 + CL_H;
+constexpr int P_B3 = P_W3 + CL_OUT * CL_H * CL_TAPS;
+constexpr int P_TOTAL = P_B3 + CL_OUT;
+static_assert(P_TOTAL == GABE_CLOSURE_NPARAMS, "parameter count of the C ABI");
+
+constexpr int WIDE_STEPS = 14;   /* 27 taps x 16 channels, 2 taps per MFMA */
+constexpr int NARROW_STEPS = 4;  /* 27 taps x 4 channels, 8 taps per MFMA */
+constexpr int CONV_BPW = 4;      /* 16-site blocks per wave */
+constexpr int WG_ROWS = 2;       /* lattice rows per weight-gradient slab */
+constexpr int WG_SLABS = 8;      /* slabs per weight-gradient workgroup */
+
+enum ConvEpi { EPI_HIDDEN_TRAIN, EPI_HIDDEN, EPI_BWD, EPI_OUT };
+
+/* packed MFMA B operands, [steps][64 lanes] x 8 BF16: the three forward
+ * layers, then the backward-data convolutions of layers 3 and 2; biases are
+ * kept as [3][16] FP32, zero padded */
+constexpr size_t PK_F1 = 0, PK_F2 = PK_F1 + NARROW_STEPS * 64, PK_F3 = PK_F2 + WIDE_STEPS * 64,
+                 PK_B3 = PK_F3 + WIDE_STEPS * 64, PK_B2 = PK_B3 + NARROW_STEPS * 64,
+                 PK_TOTAL = PK_B2 + WIDE_STEPS * 64;
+
+struct ConvArgs {
+    const bf16 *in;
+    const bf16x8 *w;
+    const float *bias;
+    int N;
+    long N3;
+    bf16 *z_out, *a_out;     /* HIDDEN: [N3][16] */
+    const bf16 *zin;         /* BWD: pre-activation of the layer below, [N3][16] */
+    bf16 *g_out;             /* BWD: [N3][16] */
+    float *y_out;            /* OUT: [N3][2] */
+    const int *gate;         /* skip everything when gate[0] == 0 */
+};
+
+__device__ __forceinline__ long site_of(int i, int j, int k, int t, int N, long N2)
+{
+    int ii = i + t / 9 - 1, jj = j + (t / 3) % 3 - 1, kk = k + t % 3 - 1;
+    ii = ii < 0 ? ii + N : (ii >= N ? ii - N : ii);
+    jj = jj < 0 ? jj + N : (jj >= N ? jj - N : jj);
+    kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
+    return (long)ii * N2 + (long)jj * N + kk;
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z)
+{
+    const float s = 1.f / (1.f + __expf(-z));
+    return s * (1.f + z * (1.f - s));
+}
+
+/* XCD-aware workgroup order: consecutive hardware workgroups go round-robin
+ * to the 8 XCDs; give each XCD a contiguous range of logical workgroups so
+ * the halo sites of neighbours share its L2 */
+__device__ __forceinline__ int logical_wg()
+{
+    const int nwg = gridDim.x, hw = blockIdx.x;
+    return nwg % GABE_NXCD == 0 ? (hw % GABE_NXCD) * (nwg / GABE_NXCD) + hw / GABE_NXCD : hw;
+}
+
+template <int CI, int EPI>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs A)
+{
+    constexpr int NSTEP = CI == 16 ? WIDE_STEPS : NARROW_STEPS;
+    __shared__ bf16x8 wl[NSTEP * 64];
+    __shared__ float bl[16];
+    if (A.gate && A.gate[0] == 0) return;
+    for (int q = threadIdx.x; q < NSTEP * 64; q += 256) wl[q] = A.w[q];
+    if (threadIdx.x < 16) bl[threadIdx.x] = A.bias ? A.bias[threadIdx.x] : 0.f;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int N = A.N;
+    const long N2 = (long)N * N;
+    const int wg = logical_wg();
+    for (int bb = 0; bb < CONV_BPW; bb++) {
+        const long s0 = (((long)wg * 4 + wave) * CONV_BPW + bb) * 16;
+        const int i = (int)(s0 / N2), j = (int)((s0 / N) % N), k0 = (int)(s0 % N);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < NSTEP; s++) {
+            bf16x8 a;
+            if constexpr (CI == 16) {
+                const int t = 2 * s + (g >> 1);
+                if (t < CL_TAPS) {
+                    a = *(const bf16x8 *)(A.in + site_of(i, j, k0 + r, t, N, N2) * 16 + 8 * (g & 1));
+                } else {
+                    for (int e = 0; e < 8; e++) a[e] = (bf16)0.f;
+                }
+            } else {
+                const int t0 = 8 * s + 2 * g;
+                bf16x4 lo, hi;
+                for (int e = 0; e < 4; e++) lo[e] = hi[e] = (bf16)0.f;
+                if (t0 < CL_TAPS) lo = *(const bf16x4 *)(A.in + site_of(i, j, k0 + r, t0, N, N2) * 4);
+                if (t0 + 1 < CL_TAPS) hi = *(const bf16x4 *)(A.in + site_of(i, j, k0 + r, t0 + 1, N, N2) * 4);
+                a = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wl[s * 64 + lane], acc, 0, 0, 0);
+        }
+        /* lane holds D[site 4g+q][channel r] */
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long site = s0 + 4 * g + q;
+            const float v = acc[q] + bl[r];
+            if constexpr (EPI == EPI_HIDDEN_TRAIN) {
+                A.z_out[site * 16 + r] = (bf16)v;
+                A.a_out[site * 16 + r] = (bf16)silu(v);
+            } else if constexpr (EPI == EPI_HIDDEN) {
+                A.a_out[site * 16 + r] = (bf16)silu(v);
+            } else if constexpr (EPI == EPI_BWD) {
+                const float z = (float)A.zin[site * 16 + r];
+                A.g_out[site * 16 + r] = (bf16)(acc[q] * dsilu(z));
+            } else if constexpr (EPI == EPI_OUT) {
+                if (r < CL_OUT) A.y_out[site * 2 + r] = v;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------- the Laplacian output */
+/* lap of both channels of u [N3][2] at site s */
+__device__ __forceinline__ float2 lap2(const float *u, long s, int N, long N2)
+{
+    const int i = (int)(s / N2), j = (int)((s / N) % N), k = (int)(s % N);
+    const long ip = (i + 1 == N ? 0 : i + 1) * N2 + (long)j * N + k, im = (i == 0 ? N - 1 : i - 1) * N2 + (long)j * N + k;
+    const long jp = (long)i * N2 + (long)(j + 1 == N ? 0 : j + 1) * N + k, jm = (long)i * N2 + (long)(j == 0 ? N - 1 : j - 1) * N + k;
+    const long kp = (long)i * N2 + (long)j * N + (k + 1 == N ? 0 : k + 1), km = (long)i * N2 + (long)j * N + (k == 0 ? N - 1 : k - 1);
+    const float2 c = *(const float2 *)(u + 2 * s);
+    float2 sum = *(const float2 *)(u + 2 * ip);
+    for (long q : {im, jp, jm, kp, km}) {
+        const float2 v = *(const float2 *)(u + 2 * q);
+        sum.x += v.x;
+        sum.y += v.y;
+    }
+    return make_float2(sum.x - 6.f * c.x, sum.y - 6.f * c.y);
+}
+
+/* the correction after a step: v_c += dt s_out[c] (lap u)_c / a^2 */
+__global__ __launch_bounds__(256) void k_lap_apply(const float *u, int N, float *v0, float *v1, const DevScalars *sc,
+                                                   float dts0, float dts1, const int *gate)
+{
+    if (gate && gate[0] == 0) return;
+    const long N2 = (long)N * N, N3 = N2 * N;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    const double a = sc->a[0];
+    const float ia2 = (float)(1. / (a * a));
+    const float2 y = lap2(u, s, N, N2);
+    v0[s] += dts0 * ia2 * y.x;
+    v1[s] += dts1 * ia2 * y.y;
+}
+
+/* loss terms (lap u - t)^2 (block partials) and dL/d(lap u) = (lap u - t) / N^3 */
+__global__ __launch_bounds__(256) void k_lap_loss(const float *u, const float *t, int N, float *gy, double *part)
+{
+    __shared__ double red[4];
+    const long N2 = (long)N * N, N3 = N2 * N;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.;
+    if (s < N3) {
+        const float2 y = lap2(u, s, N, N2);
+        const float d0 = y.x - t[s], d1 = y.y - t[N3 + s];
+        acc = (double)d0 * d0 + (double)d1 * d1;
+        const float inv = (float)(1. / (double)N3);
+        *(float2 *)(gy + 2 * s) = make_float2(d0 * inv, d1 * inv);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+/* y = lap u, both channels, [N3][2] */
+__global__ __launch_bounds__(256) void k_lap_out(const float *u, int N, float *y)
+{
+    const long N2 = (long)N * N, N3 = N2 * N;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    *(float2 *)(y + 2 * s) = lap2(u, s, N, N2);
+}
+
+/* dL/du = lap dL/d(lap u) (lap is self-adjoint), as the BF16 input [N3][4]
+ * of the backward convolution of layer 3 (channels 2, 3 zero) */
+__global__ __launch_bounds__(256) void k_lap_grad(const float *gy, int N, bf16 *g3)
+{
+    const long N2 = (long)N * N, N3 = N2 * N;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    const float2 g = lap2(gy, s, N, N2);
+    bf16x4 o;
+    o[0] = (bf16)g.x;
+    o[1] = (bf16)g.y;
+    o[2] = o[3] = (bf16)0.f;
+    *(bf16x4 *)(g3 + 4 * s) = o;
+}
+
+/* ------------------------------------------------------ weight gradient */
+struct WgradArgs {
+    const bf16 *gz;      /* [N3][gzc] */
+    const bf16 *act;     /* [N3][CI] */
+    int gzc, co;         /* channels per site of gz, channels used */
+    int N;
+    long N3;
+    float *part;         /* [nwg][NG][16][16] */
+    float *bias_part;    /* [nwg][16] */
+};
+
+/* dW[co][ci][t] = sum_s gz[s][co] act[s + off(t)][ci] and db[co] = sum_s gz[s][co]
+ * over WG_SLABS slabs (one i-plane, WG_ROWS rows, all k) per workgroup.  LDS
+ * holds gz channel-major and act channel-major in three copies shifted by
+ * dx = -1, 0, +1, so every MFMA operand is one aligned 16-byte read. */
+template <int CI>
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs A)
+{
+    constexpr int NG = CI == 16 ? CL_TAPS : 7;   /* tap groups: one tap, or four taps of 4 channels */
+    constexpr int GPW = (NG + 3) / 4;            /* groups per wave */
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = A.N;
+    const long N2 = (long)N * N;
+    const int SL = WG_ROWS * N;                  /* sites per slab */
+    const int RH = WG_ROWS + 2;
+    bf16 *gzT = (bf16 *)smem;                    /* [16][SL] */
+    bf16 *actT = gzT + 16 * SL;                  /* [3 dx][CI][3 planes][RH][N] */
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int wg = logical_wg();
+
+    f32x4 acc[GPW];
+    for (int q = 0; q < GPW; q++) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    const int slabs_per_plane = N / WG_ROWS;
+    for (int sl = 0; sl < WG_SLABS; sl++) {
+        const int slab = wg * WG_SLABS + sl;
+        const int i = slab / slabs_per_plane, j0 = (slab % slabs_per_plane) * WG_ROWS;
+        __syncthreads();
+        /* stage gz (channel-major) */
+        for (int q = threadIdx.x; q < SL; q += 256) {
+            const int row = q / N, k = q % N;
+            const long site = (long)i * N2 + (long)(j0 + row) * N + k;
+            for (int c = 0; c < 16; c++)
+                gzT[c * SL + q] = c < A.co ? A.gz[site * A.gzc + c] : (bf16)0.f;
+        }
+        /* stage act: planes i-1..i+1, rows j0-1..j0+WG_ROWS, three dx-shifted copies */
+        for (int q = threadIdx.x; q < 3 * RH * N; q += 256) {
+            const int pl = q / (RH * N), rr = (q / N) % RH, k = q % N;
+            int ii = i + pl - 1, jj = j0 + rr - 1;
+            ii = ii < 0 ? ii + N : (ii >= N ? ii - N : ii);
+            jj = jj < 0 ? jj + N : (jj >= N ? jj - N : jj);
+            const long site = (long)ii * N2 + (long)jj * N + k;
+            bf16 v[CI];
+            if constexpr (CI == 16) {
+                *(bf16x8 *)v = *(const bf16x8 *)(A.act + site * 16);
+                *(bf16x8 *)(v + 8) = *(const bf16x8 *)(A.act + site * 16 + 8);
+            } else {
+                *(bf16x4 *)v = *(const bf16x4 *)(A.act + site * 4);
+            }
+            /* copy dxi holds act at k' = k + dx, so site k lands at k - dx */
+            for (int dxi = 0; dxi < 3; dxi++) {
+                int kd = k - (dxi - 1);
+                kd = kd < 0 ? kd + N : (kd >= N ? kd - N : kd);
+                for (int c = 0; c < CI; c++) actT[(((dxi * CI + c) * 3 + pl) * RH + rr) * N + kd] = v[c];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 16)
+            for (int q = 0; q < SL; q++) bsum += (float)gzT[threadIdx.x * SL + q];
+        for (int kk = 0; kk < SL / 32; kk++) {
+            const int q0 = kk * 32;
+            const int row = q0 / N, kb = q0 % N + 8 * g;
+            const bf16x8 av = *(const bf16x8 *)(gzT + r * SL + q0 + 8 * g);
+#pragma unroll
+            for (int gi = 0; gi < GPW; gi++) {
+                const int grp = wave + 4 * gi;
+                if (grp >= NG) break;
+                int t, c;
+                if constexpr (CI == 16) {
+                    t = grp;
+                    c = r;
+                } else {
+                    t = 4 * grp + (r >> 2);
+                    c = r & 3;
+                }
+                bf16x8 bv;
+                if (t < CL_TAPS) {
+                    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+                    bv = *(const bf16x8 *)(actT + (((dx * CI + c) * 3 + dz) * RH + row + dy) * N + kb);
+                } else {
+                    for (int e = 0; e < 8; e++) bv[e] = (bf16)0.f;
+                }
+                acc[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[gi], 0, 0, 0);
+            }
+        }
+    }
+    /* D[co = 4g+q][n = r] of group grp */
+    for (int gi = 0; gi < GPW; gi++) {
+        const int grp = wave + 4 * gi;
+        if (grp >= NG) break;
+        for (int q = 0; q < 4; q++) A.part[(((long)wg * NG + grp) * 16 + 4 * g + q) * 16 + r] = acc[gi][q];
+    }
+    if (threadIdx.x < 16) A.bias_part[(long)wg * 16 + threadIdx.x] = bsum;
+}
+
+/* grad[param] = sum over workgroups, fixed order.  layer 1: W1[co][ci][t] from
+ * group t/4, column 4 (t%4) + ci; layers 2/3: W[co][ci][t] from group t,
+ * column ci. */
+__global__ void k_wgrad_reduce(const float *part, const float *bias_part, int nwg, int NG, int CO, int CI,
+                               float *gw, float *gb)
+{
+    const int nw = CO * CI * CL_TAPS;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nw) {
+        const int co = e / (CI * CL_TAPS), ci = (e / CL_TAPS) % CI, t = e % CL_TAPS;
+        const int grp = CI == 4 ? t / 4 : t, col = CI == 4 ? 4 * (t % 4) + ci : ci;
+        double s = 0.;
+        for (int w = 0; w < nwg; w++) s += part[(((long)w * NG + grp) * 16 + co) * 16 + col];
+        gw[e] = (float)s;
+    } else if (e < nw + CO) {
+        const int co = e - nw;
+        double s = 0.;
+        for (int w = 0; w < nwg; w++) s += bias_part[(long)w * 16 + co];
+        gb[co] = (float)s;
+    }
+}
+
+/* ----------------------------------------------------------- packing */
+/* B operand value of a conv with input channels cin (padded to 16 or 4 slots)
+ * and output channels cout, from W[co][ci][t] (or, flipped, from the layer
+ * whose weights are Wf[ci][co][26 - t]). */
+__device__ float packed_weight(const float *W, int wco, int wci, bool flip, int wide, int s, int lane, int e)
+{
+    const int n = lane & 15, g = lane >> 4;
+    int t, ci;
+    if (wide) {
+        t = 2 * s + (g >> 1);
+        ci = 8 * (g & 1) + e;
+    } else {
+        t = 8 * s + 2 * g + (e >> 2);
+        ci = e & 3;
+    }
+    if (t >= CL_TAPS) return 0.f;
+    if (!flip) {   /* conv from wci channels to wco channels */
+        if (n >= wco || ci >= wci) return 0.f;
+        return W[(n * wci + ci) * CL_TAPS + t];
+    }
+    /* transposed: input channels are the layer's outputs (wco), outputs its inputs (wci) */
+    if (n >= wci || ci >= wco) return 0.f;
+    return W[(ci * wci + n) * CL_TAPS + (CL_TAPS - 1 - t)];
+}
+
+__global__ void k_pack(const float *P, bf16x8 *pk, float *bias)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < (long)PK_TOTAL) {
+        const float *W;
+        int wco, wci, wide, s;
+        bool flip;
+        long off;
+        if (q < (long)PK_F2) { W = P + P_W1; wco = 16; wci = 4; flip = false; wide = 0; off = PK_F1; }
+        else if (q < (long)PK_F3) { W = P + P_W2; wco = 16; wci = 16; flip = false; wide = 1; off = PK_F2; }
+        else if (q < (long)PK_B3) { W = P + P_W3; wco = 2; wci = 16; flip = false; wide = 1; off = PK_F3; }
+        else if (q < (long)PK_B2) { W = P + P_W3; wco = 2; wci = 16; flip = true; wide = 0; off = PK_B3; }
+        else { W = P + P_W2; wco = 16; wci = 16; flip = true; wide = 1; off = PK_B2; }
+        s = (int)((q - off) / 64);
+        const int lane = (int)((q - off) % 64);
+        bf16x8 v;
+        for (int e = 0; e < 8; e++) v[e] = (bf16)packed_weight(W, wco, wci, flip, wide, s, lane, e);
+        pk[q] = v;
+    }
+    if (q < 48) {
+        const int l = (int)(q / 16), c = (int)(q % 16);
+        const int off = l == 0 ? P_B1 : (l == 1 ? P_B2 : P_B3), n = l == 2 ? CL_OUT : CL_H;
+        bias[q] = c < n ? P[off + c] : 0.f;
+    }
+}
+
+__global__ void k_adam(float *P, const float *G, float *m, float *v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float g = G[e];
+    const float mm = b1 * m[e] + (1.f - b1) * g;
+    const float vv = b2 * v[e] + (1.f - b2) * g * g;
+    m[e] = mm;
+    v[e] = vv;
+    P[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+/* ------------------------------------------------------ samples */
+__global__ void k_make_x(const float *f0, const float *f1, const float *v0, const float *v1, float i0, float i1,
+                         float i2, float i3, long N3, bf16 *x, const int *gate)
+{
+    if (gate && gate[0] == 0) return;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    bf16x4 o;
+    o[0] = (bf16)(f0[s] * i0);
+    o[1] = (bf16)(f1[s] * i1);
+    o[2] = (bf16)(v0[s] * i2);
+    o[3] = (bf16)(v1[s] * i3);
+    *(bf16x4 *)(x + s * 4) = o;
+}
+
+/* 7-point lattice Laplacian of the two fluctuation fields f [2][N3]:
+ * out_c = (sum of the 6 neighbours - 6 f_c) / dx^2 */
+__device__ __forceinline__ float lap_soa(const float *f, long s, int N, long N2)
+{
+    const int i = (int)(s / N2), j = (int)((s / N) % N), k = (int)(s % N);
+    const long base = s - k;
+    float sum = f[(i + 1 == N ? s - (long)i * N2 : s + N2)] + f[(i == 0 ? s + (long)(N - 1) * N2 : s - N2)];
+    sum += f[(j + 1 == N ? s - (long)j * N : s + N)] + f[(j == 0 ? s + (long)(N - 1) * N : s - N)];
+    sum += f[base + (k + 1 == N ? 0 : k + 1)] + f[base + (k == 0 ? N - 1 : k - 1)];
+    return sum - 6.f * f[s];
+}
+
+__global__ __launch_bounds__(256) void k_lap_field(const float *f, int N, float inv_dx2, float *out)
+{
+    const long N2 = (long)N * N, N3 = N2 * N;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    out[s] = lap_soa(f, s, N, N2) * inv_dx2;
+    out[N3 + s] = lap_soa(f + N3, s, N, N2) * inv_dx2;
+}
+
+/* target t_c <- (t_c - lap f_c / dx^2) * scale_c: the restricted fine
+ * Laplacian minus the coarse Laplacian of the restricted field */
+__global__ __launch_bounds__(256) void k_lap_target(const float *f, int N, float inv_dx2, float sc0, float sc1, float *t)
+{
+    const long N2 = (long)N * N, N3 = N2 * N;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    t[s] = (t[s] - lap_soa(f, s, N, N2) * inv_dx2) * sc0;
+    t[N3 + s] = (t[N3 + s] - lap_soa(f + N3, s, N, N2) * inv_dx2) * sc1;
+}
+
+__global__ __launch_bounds__(256) void k_scale2(float *t, long N3, float sc0, float sc1)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    t[s] *= sc0;
+    t[N3 + s] *= sc1;
+}
+
+__global__ void k_sumsq(const float *x, long n, double *part)
+{
+    __shared__ double red[4];
+    double acc = 0.;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        acc += (double)x[q] * x[q];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+/* Friedmann-constraint gate (on the device, no host round trip): from the
+ * last three history rows, R = |rho_dot + 3H(rho + p)| / (3 H rho) at the
+ * middle row with p = K - G/3 - V; the closure stops for good once R exceeds
+ * the threshold or a value is not finite. */
+__global__ void k_gate(const DevScalars *sc, const double *hist, int hist_cap, double thr, int *gate,
+                       long long *fallback, double *rmax)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || gate[0] == 0) return;
+    const long long n = sc->nstep - 1;
+    if (n < 2) return;
+    const double *rm = hist + ((n - 2) % hist_cap) * GABE_HIST_COLS;
+    const double *r0 = hist + ((n - 1) % hist_cap) * GABE_HIST_COLS;
+    const double *rp = hist + (n % hist_cap) * GABE_HIST_COLS;
+    const double H = r0[2] / r0[1], rho = r0[6], p = r0[3] - r0[5] / 3. - r0[4];
+    const double rhodot = (rp[6] - rm[6]) / (rp[0] - rm[0]);
+    const double R = fabs(rhodot + 3. * H * (rho + p)) / (3. * H * rho);
+    if (R > rmax[0]) rmax[0] = R;
+    if (!(R <= thr)) {
+        gate[0] = 0;
+        fallback[0] = sc->nstep;
+    }
+}
+
+/* coarse half-spectrum <- fine half-spectrum on |n_a| < Nc/2, times scale;
+ * zero on the coarse Nyquist planes */
+template <typename T>
+__global__ void k_restrict_modes(const T *fine, int Nf, T *coarse, int Nc, T scale)
+{
+    const int Hc = Nc / 2 + 1, Hf = Nf / 2 + 1;
+    const long Mc = (long)Nc * Nc * Hc;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= Mc) return;
+    const int kc = (int)(m % Hc), jc = (int)((m / Hc) % Nc), ic = (int)(m / ((long)Hc * Nc));
+    T re = 0, im = 0;
+    if (ic != Nc / 2 && jc != Nc / 2 && kc != Nc / 2) {
+        const int i_f = ic < Nc / 2 ? ic : Nf - (Nc - ic), j_f = jc < Nc / 2 ? jc : Nf - (Nc - jc);
+        const long mf = ((long)i_f * Nf + j_f) * Hf + kc;
+        re = fine[2 * mf] * scale;
+        im = fine[2 * mf + 1] * scale;
+    }
+    coarse[2 * m] = re;
+    coarse[2 * m + 1] = im;
+}
+
+inline unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+/* deterministic initial weights: uniform in +-sqrt(3/fan_in) from a 64-bit
+ * LCG, biases zero, the output layer scaled down so training starts near the
+ * plain step */
+void init_params(std::vector<float> &P, unsigned seed)
+{
+    unsigned long long st = 0x9E3779B97F4A7C15ULL ^ ((unsigned long long)seed << 1);
+    auto uni = [&]() {
+        st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    P.assign(P_TOTAL, 0.f);
+    const struct { int off, n, fan; double gain; } L[3] = {
+        {P_W1, CL_H * CL_IN * CL_TAPS, CL_IN * CL_TAPS, 1.0},
+        {P_W2, CL_H * CL_H * CL_TAPS, CL_H * CL_TAPS, 1.0},
+        {P_W3, CL_OUT * CL_H * CL_TAPS, CL_H * CL_TAPS, 0.1},
+    };
+    for (const auto &l : L) {
+        const double a = l.gain * std::sqrt(3.0 / l.fan);
+        for (int e = 0; e < l.n; e++) P[l.off + e] = (float)(a * uni());
+    }
+}
+
+} // namespace
+
+/* =================================================================== state */
+struct gabe_closure {
+    int N = 0;
+    long N3 = 0;
+    double dt = 0.;
+    hipStream_t stream = nullptr;
+    double s_in[4] = {0, 0, 0, 0}, s_out[2] = {0, 0};
+    bool scaled = false;
+    float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;
+    long long adam_t = 0;
+    bf16x8 *pk = nullptr;
+    float *bias = nullptr;
+    bf16 *z1 = nullptr, *a1 = nullptr, *z2 = nullptr, *a2 = nullptr, *g3 = nullptr, *gz2 = nullptr, *gz1 = nullptr;
+    bf16 *xbuf = nullptr;
+    float *tbuf = nullptr, *ybuf = nullptr, *gybuf = nullptr, *wbuf = nullptr;
+    float *wpart = nullptr, *bpart = nullptr;
+    double *lpart = nullptr, *red = nullptr;
+    float *flap = nullptr;      /* Laplacian of the fine lattice's fields [2][N3 fine] */
+    long flap_n3 = 0;
+    std::vector<bf16 *> xs;     /* dataset: inputs [N3][4] */
+    std::vector<float *> ts;    /* dataset: targets [2][N3] */
+    int conv_wg = 0, wgrad_wg = 0;
+};
+
+struct ClosureRun {
+    int N = 0;                     /* weights and scales are copies: the closure may go */
+    bf16x8 *pk = nullptr;
+    float *bias = nullptr;
+    bf16 *x = nullptr, *a1 = nullptr, *a2 = nullptr;
+    float *u = nullptr, *w = nullptr;   /* network output u and lap u, [N3][2] */
+    int *gate = nullptr;           /* [0] active */
+    long long *fallback = nullptr; /* step count when the gate closed */
+    double *rmax = nullptr;
+    double thr = 0.;
+    float inv_s[4], dt_s[2];
+};
+
+namespace {
+
+int cl_check_n(int N)
+{
+    if (N < 32 || N > 128 || N % 32) return fail(GABE_ERR_ARG, "closure lattices need N in {32, 64, 96, 128} (got %d)", N);
+    return GABE_OK;
+}
+
+/* one workgroup per 4 waves x CONV_BPW blocks of 16 sites */
+int conv_workgroups(long N3) { return (int)(N3 / (16 * 4 * CONV_BPW)); }
+
+template <int CI, int EPI>
+int launch_conv(const ConvArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_conv<CI, EPI>), dim3(conv_workgroups(a.N3)), dim3(256), 0, s, a);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+ConvArgs conv_base(int N, const bf16 *in, const bf16x8 *w, const float *bias)
+{
+    ConvArgs a{};
+    a.in = in;
+    a.w = w;
+    a.bias = bias;
+    a.N = N;
+    a.N3 = (long)N * N * N;
+    return a;
+}
+
+size_t wgrad_lds(int CI, int N) { return (size_t)2 * (16 * WG_ROWS * N + 3 * CI * 3 * (WG_ROWS + 2) * N); }
+
+template <int CI>
+int launch_wgrad(gabe_closure *cl, const bf16 *gz, int gzc, int co, const bf16 *act, float *gw, float *gb)
+{
+    WgradArgs w{gz, act, gzc, co, cl->N, cl->N3, cl->wpart, cl->bpart};
+    const size_t lds = wgrad_lds(CI, cl->N);
+    hipLaunchKernelGGL((k_wgrad<CI>), dim3(cl->wgrad_wg), dim3(256), lds, cl->stream, w);
+    HIPCHK(hipGetLastError());
+    const int NG = CI == 16 ? CL_TAPS : 7, CIw = CI == 16 ? 16 : 4;
+    const int n = co * CIw * CL_TAPS + co;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblocks(n, 256)), dim3(256), 0, cl->stream, cl->wpart, cl->bpart,
+                       cl->wgrad_wg, NG, co, CIw, gw, gb);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+int repack(gabe_closure *cl)
+{
+    hipLaunchKernelGGL(k_pack, dim3(nblocks((long)PK_TOTAL, 256)), dim3(256), 0, cl->stream, cl->P, cl->pk,
+                       cl->bias);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+/* forward (training form: keeps z and a of both hidden layers) and the loss
+ * against targets t, leaving dL/dy in g3 */
+int forward_loss(gabe_closure *cl, const bf16 *x, const float *t)
+{
+    ConvArgs a = conv_base(cl->N, x, cl->pk + PK_F1, cl->bias);
+    a.z_out = cl->z1;
+    a.a_out = cl->a1;
+    CHK((launch_conv<4, EPI_HIDDEN_TRAIN>(a, cl->stream)));
+    a = conv_base(cl->N, cl->a1, cl->pk + PK_F2, cl->bias + 16);
+    a.z_out = cl->z2;
+    a.a_out = cl->a2;
+    CHK((launch_conv<16, EPI_HIDDEN_TRAIN>(a, cl->stream)));
+    a = conv_base(cl->N, cl->a2, cl->pk + PK_F3, cl->bias + 32);
+    a.y_out = cl->ybuf;
+    CHK((launch_conv<16, EPI_OUT>(a, cl->stream)));
+    /* y = lap lap u; dL/du = lap lap dL/dy (lap is self-adjoint) */
+    const unsigned nb = nblocks(cl->N3, 256);
+    hipLaunchKernelGGL(k_lap_out, dim3(nb), dim3(256), 0, cl->stream, cl->ybuf, cl->N, cl->wbuf);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lap_loss, dim3(nb), dim3(256), 0, cl->stream, cl->wbuf, t, cl->N, cl->gybuf, cl->lpart);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lap_out, dim3(nb), dim3(256), 0, cl->stream, cl->gybuf, cl->N, cl->wbuf);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lap_grad, dim3(nb), dim3(256), 0, cl->stream, cl->wbuf, cl->N, cl->g3);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+/* gradients of the loss left by forward_loss, into cl->G */
+int backward(gabe_closure *cl, const bf16 *x)
+{
+    CHK(launch_wgrad<16>(cl, cl->g3, 4, CL_OUT, cl->a2, cl->G + P_W3, cl->G + P_B3));
+    ConvArgs a = conv_base(cl->N, cl->g3, cl->pk + PK_B3, nullptr);
+    a.zin = cl->z2;
+    a.g_out = cl->gz2;
+    CHK((launch_conv<4, EPI_BWD>(a, cl->stream)));
+    CHK(launch_wgrad<16>(cl, cl->gz2, 16, CL_H, cl->a1, cl->G + P_W2, cl->G + P_B2));
+    a = conv_base(cl->N, cl->gz2, cl->pk + PK_B2, nullptr);
+    a.zin = cl->z1;
+    a.g_out = cl->gz1;
+    CHK((launch_conv<16, EPI_BWD>(a, cl->stream)));
+    CHK(launch_wgrad<4>(cl, cl->gz1, 16, CL_H, x, cl->G + P_W1, cl->G + P_B1));
+    return GABE_OK;
+}
+
+int read_loss(gabe_closure *cl, double *loss)
+{
+    std::vector<double> lp(nblocks(cl->N3, 256));
+    HIPCHK(hipMemcpyAsync(lp.data(), cl->lpart, sizeof(double) * lp.size(), hipMemcpyDeviceToHost, cl->stream));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    *loss = csum(lp.data(), (long)lp.size()) / (2.0 * (double)cl->N3);
+    return GABE_OK;
+}
+
+int rms_of(gabe_closure *cl, const float *x, long n, double *out)
+{
+    const int nb = 512;
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, cl->stream, x, n, cl->red);
+    HIPCHK(hipGetLastError());
+    std::vector<double> p(nb);
+    HIPCHK(hipMemcpyAsync(p.data(), cl->red, sizeof(double) * nb, hipMemcpyDeviceToHost, cl->stream));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    *out = std::sqrt(csum(p.data(), nb) / (double)n);
+    return GABE_OK;
+}
+
+void free_closure(gabe_closure *cl)
+{
+    if (cl->stream) (void)hipStreamSynchronize(cl->stream);
+    void *bufs[] = {cl->P, cl->G, cl->m, cl->v, cl->pk, cl->bias, cl->z1, cl->a1, cl->z2, cl->a2, cl->g3, cl->gz2,
+                    cl->gz1, cl->xbuf, cl->tbuf, cl->ybuf, cl->gybuf, cl->wbuf, cl->wpart, cl->bpart, cl->lpart, cl->red,
+                    cl->flap};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto *p : cl->xs) (void)hipFree(p);
+    for (auto *p : cl->ts) (void)hipFree(p);
+    if (cl->stream) (void)hipStreamDestroy(cl->stream);
+}
+
+int create_closure(gabe_closure *cl, int N, double dt, unsigned seed)
+{
+    cl->N = N;
+    cl->N3 = (long)N * N * N;
+    cl->dt = dt;
+    cl->conv_wg = conv_workgroups(cl->N3);
+    cl->wgrad_wg = (int)((long)N * (N / WG_ROWS) / WG_SLABS);
+    HIPCHK(hipStreamCreate(&cl->stream));
+    const long N3 = cl->N3;
+    HIPCHK(hipMalloc(&cl->P, sizeof(float) * P_TOTAL));
+    HIPCHK(hipMalloc(&cl->G, sizeof(float) * P_TOTAL));
+    HIPCHK(hipMalloc(&cl->m, sizeof(float) * P_TOTAL));
+    HIPCHK(hipMalloc(&cl->v, sizeof(float) * P_TOTAL));
+    HIPCHK(hipMalloc(&cl->pk, sizeof(bf16x8) * PK_TOTAL));
+    HIPCHK(hipMalloc(&cl->bias, sizeof(float) * 48));
+    for (bf16 **b : {&cl->z1, &cl->a1, &cl->z2, &cl->a2, &cl->gz2, &cl->gz1}) HIPCHK(hipMalloc(b, sizeof(bf16) * 16 * N3));
+    HIPCHK(hipMalloc(&cl->g3, sizeof(bf16) * 4 * N3));
+    HIPCHK(hipMalloc(&cl->xbuf, sizeof(bf16) * 4 * N3));
+    HIPCHK(hipMalloc(&cl->tbuf, sizeof(float) * 2 * N3));
+    HIPCHK(hipMalloc(&cl->ybuf, sizeof(float) * 2 * N3));
+    HIPCHK(hipMalloc(&cl->gybuf, sizeof(float) * 2 * N3));
+    HIPCHK(hipMalloc(&cl->wbuf, sizeof(float) * 2 * N3));
+    HIPCHK(hipMalloc(&cl->wpart, sizeof(float) * (size_t)cl->wgrad_wg * CL_TAPS * 256));
+    HIPCHK(hipMalloc(&cl->bpart, sizeof(float) * (size_t)cl->wgrad_wg * 16));
+    HIPCHK(hipMalloc(&cl->lpart, sizeof(double) * nblocks(N3, 256)));
+    HIPCHK(hipMalloc(&cl->red, sizeof(double) * 512));
+    HIPCHK(hipMemset(cl->m, 0, sizeof(float) * P_TOTAL));
+    HIPCHK(hipMemset(cl->v, 0, sizeof(float) * P_TOTAL));
+    HIPCHK(hipFuncSetAttribute((const void *)k_wgrad<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)wgrad_lds(16, N)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_wgrad<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)wgrad_lds(4, N)));
+    std::vector<float> P;
+    init_params(P, seed);
+    HIPCHK(hipMemcpy(cl->P, P.data(), sizeof(float) * P_TOTAL, hipMemcpyHostToDevice));
+    CHK(repack(cl));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    return GABE_OK;
+}
+
+bool fp32_pair(const gabe_ctx *c) { return c->fp32 && c->nf == 2; }
+
+/* restriction of one array of the fine context into one of the coarse */
+int restrict_array(gabe_ctx *f, gabe_ctx *c, void *src, void *dst)
+{
+    CHK(run_plan(f, f->fft_rx, src, f->rx_buf));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    const long Mc = (long)c->N * c->N * (c->N / 2 + 1);
+    const double scale = 1.0 / (double)f->N3;
+    if (c->fp32)
+        hipLaunchKernelGGL(k_restrict_modes<float>, dim3(nblocks(Mc, 256)), dim3(256), 0, c->stream,
+                           (const float *)f->rx_buf, f->N, (float *)c->rx_buf, c->N, (float)scale);
+    else
+        hipLaunchKernelGGL(k_restrict_modes<double>, dim3(nblocks(Mc, 256)), dim3(256), 0, c->stream,
+                           (const double *)f->rx_buf, f->N, (double *)c->rx_buf, c->N, scale);
+    HIPCHK(hipGetLastError());
+    CHK(run_plan(c, c->fft_rx, c->rx_buf, dst));
+    return GABE_OK;
+}
+
+} // namespace
+
+namespace gabe_detail {
+
+int closure_after_step(gabe_ctx *c)
+{
+    ClosureRun *R = c->closure;
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, c->stream, c->sc, c->hist, c->hist_cap, R->thr, R->gate,
+                       R->fallback, R->rmax);
+    HIPCHK(hipGetLastError());
+    const float *f = (const float *)c->fbuf[c->cur], *v = (const float *)c->vbuf[c->cur];
+    hipLaunchKernelGGL(k_make_x, dim3(nblocks(c->N3, 256)), dim3(256), 0, c->stream, f, f + c->N3, v, v + c->N3,
+                       R->inv_s[0], R->inv_s[1], R->inv_s[2], R->inv_s[3], c->N3, R->x, R->gate);
+    HIPCHK(hipGetLastError());
+    ConvArgs a = conv_base(R->N, R->x, R->pk + PK_F1, R->bias);
+    a.gate = R->gate;
+    a.a_out = R->a1;
+    CHK((launch_conv<4, EPI_HIDDEN>(a, c->stream)));
+    a = conv_base(R->N, R->a1, R->pk + PK_F2, R->bias + 16);
+    a.gate = R->gate;
+    a.a_out = R->a2;
+    CHK((launch_conv<16, EPI_HIDDEN>(a, c->stream)));
+    a = conv_base(R->N, R->a2, R->pk + PK_F3, R->bias + 32);
+    a.gate = R->gate;
+    a.y_out = R->u;
+    CHK((launch_conv<16, EPI_OUT>(a, c->stream)));
+    float *vw = (float *)c->vbuf[c->cur];
+    hipLaunchKernelGGL(k_lap_out, dim3(nblocks(c->N3, 256)), dim3(256), 0, c->stream, R->u, c->N, R->w);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lap_apply, dim3(nblocks(c->N3, 256)), dim3(256), 0, c->stream, R->w, c->N, vw, vw + c->N3,
+                       c->sc, R->dt_s[0], R->dt_s[1], R->gate);
+    HIPCHK(hipGetLastError());
+    return GABE_OK;
+}
+
+void closure_release(gabe_ctx *c)
+{
+    ClosureRun *R = c->closure;
+    if (!R) return;
+    void *bufs[] = {R->pk, R->bias, R->x, R->a1, R->a2, R->u, R->w, R->gate, R->fallback, R->rmax};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete R;
+    c->closure = nullptr;
+}
+
+} // namespace gabe_detail
+
+/* ====================================================================== ABI */
+namespace {
+
+/* checks and the transforms of a fine -> coarse restriction */
+int restrict_setup(gabe_ctx *fine, gabe_ctx *coarse)
+{
+    if (fine->nf != coarse->nf || fine->fp32 != coarse->fp32 || fine->p.L != coarse->p.L)
+        return fail(GABE_ERR_ARG, "restriction needs the same nflds, precision and box size L");
+    if (coarse->N > fine->N || (coarse->N & 1)) return fail(GABE_ERR_ARG, "restriction needs an even coarse N <= fine N");
+    const bool dbl = !fine->fp32;
+    const size_t cs = 2 * fine->esz;
+    if (!fine->fft_rx.plan) {
+        CHK(make_plan(fine, fine->fft_rx, rocfft_transform_type_real_forward, dbl, 1));
+        HIPCHK(hipMalloc(&fine->rx_buf, cs * fine->M));
+    }
+    if (!coarse->fft_rx.plan) {
+        CHK(make_plan(coarse, coarse->fft_rx, rocfft_transform_type_real_inverse, dbl, 1));
+        HIPCHK(hipMalloc(&coarse->rx_buf, cs * coarse->M));
+    }
+    return GABE_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int gabe_restrict(gabe_ctx *fine, gabe_ctx *coarse)
+{
+    if (!fine || !coarse) return fail(GABE_ERR_ARG, "NULL context");
+    CHK(restrict_setup(fine, coarse));
+    HIPCHK(hipStreamSynchronize(fine->stream));
+    HIPCHK(hipStreamSynchronize(coarse->stream));
+    for (int a = 0; a < 2; a++) {
+        char *src = (char *)(a ? fine->vbuf[fine->cur] : fine->fbuf[fine->cur]);
+        char *dst = (char *)(a ? coarse->vbuf[coarse->cur] : coarse->fbuf[coarse->cur]);
+        for (int f = 0; f < fine->nf; f++)
+            CHK(restrict_array(fine, coarse, src + fine->esz * f * fine->N3, dst + coarse->esz * f * coarse->N3));
+    }
+    HIPCHK(hipStreamSynchronize(coarse->stream));
+    DevScalars d;
+    CHK(read_scalars(fine, d));
+    DevScalars dc;
+    CHK(read_scalars(coarse, dc));
+    d.nstep = dc.nstep;    /* the coarse context keeps its own step count and history */
+    d.e0_valid = 0;
+    CHK(write_scalars(coarse, d));
+    return GABE_OK;
+}
+
+int gabe_closure_create(int N, double dt, unsigned seed, gabe_closure **out)
+{
+    if (!out) return fail(GABE_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    CHK(cl_check_n(N));
+    if (!(dt > 0.)) return fail(GABE_ERR_ARG, "dt must be positive");
+    gabe_closure *cl = new gabe_closure;
+    const int rc = create_closure(cl, N, dt, seed);
+    if (rc != GABE_OK) {
+        free_closure(cl);
+        delete cl;
+        return rc;
+    }
+    *out = cl;
+    return GABE_OK;
+}
+
+int gabe_closure_destroy(gabe_closure *cl)
+{
+    if (!cl) return GABE_OK;
+    free_closure(cl);
+    delete cl;
+    return GABE_OK;
+}
+
+int gabe_closure_params(gabe_closure *cl, float *out)
+{
+    if (!cl || !out) return fail(GABE_ERR_ARG, "NULL argument");
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    HIPCHK(hipMemcpy(out, cl->P, sizeof(float) * P_TOTAL, hipMemcpyDeviceToHost));
+    return GABE_OK;
+}
+
+int gabe_closure_set_params(gabe_closure *cl, const float *in)
+{
+    if (!cl || !in) return fail(GABE_ERR_ARG, "NULL argument");
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    HIPCHK(hipMemcpy(cl->P, in, sizeof(float) * P_TOTAL, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(cl->m, 0, sizeof(float) * P_TOTAL));
+    HIPCHK(hipMemset(cl->v, 0, sizeof(float) * P_TOTAL));
+    cl->adam_t = 0;
+    CHK(repack(cl));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    return GABE_OK;
+}
+
+int gabe_closure_scales(gabe_closure *cl, double *s_in, double *s_out)
+{
+    if (!cl || !s_in || !s_out) return fail(GABE_ERR_ARG, "NULL argument");
+    for (int c = 0; c < 4; c++) s_in[c] = cl->s_in[c];
+    for (int c = 0; c < 2; c++) s_out[c] = cl->s_out[c];
+    return GABE_OK;
+}
+
+int gabe_closure_set_scales(gabe_closure *cl, const double *s_in, const double *s_out)
+{
+    if (!cl || !s_in || !s_out) return fail(GABE_ERR_ARG, "NULL argument");
+    for (int c = 0; c < 4; c++)
+        if (!(s_in[c] > 0.)) return fail(GABE_ERR_ARG, "input scales must be positive");
+    for (int c = 0; c < 2; c++)
+        if (!(s_out[c] > 0.)) return fail(GABE_ERR_ARG, "output scales must be positive");
+    for (int c = 0; c < 4; c++) cl->s_in[c] = s_in[c];
+    for (int c = 0; c < 2; c++) cl->s_out[c] = s_out[c];
+    cl->scaled = true;
+    return GABE_OK;
+}
+
+int gabe_closure_sample(gabe_closure *cl, gabe_ctx *fine, gabe_ctx *coarse, int fine_steps, int restrict_first)
+{
+    if (!cl || !fine || !coarse) return fail(GABE_ERR_ARG, "NULL argument");
+    if (coarse->N != cl->N || !fp32_pair(coarse) || !fp32_pair(fine))
+        return fail(GABE_ERR_ARG, "samples need FP32 two-field contexts, the coarse one of the closure's N");
+    if (fine_steps < 0) return fail(GABE_ERR_ARG, "fine_steps must be >= 0");
+    CHK(restrict_setup(fine, coarse));
+    if (restrict_first) CHK(gabe_restrict(fine, coarse));
+    const long N3 = cl->N3, N3f = fine->N3;
+    HIPCHK(hipStreamSynchronize(coarse->stream));
+    HIPCHK(hipStreamSynchronize(fine->stream));
+    /* input scales from the first sample: rms of each fluctuation array */
+    const float *cf = (const float *)coarse->fbuf[coarse->cur], *cv = (const float *)coarse->vbuf[coarse->cur];
+    if (!cl->scaled) {
+        const float *src[4] = {cf, cf + N3, cv, cv + N3};
+        for (int c = 0; c < 4; c++) {
+            CHK(rms_of(cl, src[c], N3, &cl->s_in[c]));
+            if (!(cl->s_in[c] > 0.)) return fail(GABE_ERR_STATE, "zero fluctuation in input channel %d", c);
+        }
+    }
+    if (cl->flap_n3 != N3f) {
+        if (cl->flap) HIPCHK(hipFree(cl->flap));
+        cl->flap = nullptr;
+        HIPCHK(hipMalloc(&cl->flap, sizeof(float) * 2 * N3f));
+        cl->flap_n3 = N3f;
+    }
+    bf16 *x = nullptr;
+    float *t = nullptr;
+    HIPCHK(hipMalloc(&x, sizeof(bf16) * 4 * N3));
+    if (hipMalloc(&t, sizeof(float) * 2 * N3) != hipSuccess) {
+        (void)hipFree(x);
+        return fail(GABE_ERR_HIP, "sample allocation failed");
+    }
+    cl->xs.push_back(x);
+    cl->ts.push_back(t);
+    hipLaunchKernelGGL(k_make_x, dim3(nblocks(N3, 256)), dim3(256), 0, cl->stream, cf, cf + N3, cv, cv + N3,
+                       (float)(1. / cl->s_in[0]), (float)(1. / cl->s_in[1]), (float)(1. / cl->s_in[2]),
+                       (float)(1. / cl->s_in[3]), N3, x, (const int *)nullptr);
+    HIPCHK(hipGetLastError());
+    /* target: restriction of the fine lattice's Laplacian minus the coarse
+     * lattice's Laplacian of the restricted field */
+    hipLaunchKernelGGL(k_lap_field, dim3(nblocks(N3f, 256)), dim3(256), 0, fine->stream,
+                       (const float *)fine->fbuf[fine->cur], fine->N, (float)(1. / (fine->dx * fine->dx)), cl->flap);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(fine->stream));
+    for (int c = 0; c < 2; c++) CHK(restrict_array(fine, coarse, cl->flap + c * N3f, t + c * N3));
+    const bool first = !cl->scaled;
+    const float s0 = first ? 1.f : (float)(1. / cl->s_out[0]), s1 = first ? 1.f : (float)(1. / cl->s_out[1]);
+    hipLaunchKernelGGL(k_lap_target, dim3(nblocks(N3, 256)), dim3(256), 0, coarse->stream, cf, cl->N,
+                       (float)(1. / (coarse->dx * coarse->dx)), s0, s1, t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(coarse->stream));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    if (first) {   /* output scales: rms of the first raw target */
+        for (int c = 0; c < 2; c++) {
+            CHK(rms_of(cl, t + c * N3, N3, &cl->s_out[c]));
+            if (!(cl->s_out[c] > 0.)) return fail(GABE_ERR_STATE, "zero target in output channel %d", c);
+        }
+        hipLaunchKernelGGL(k_scale2, dim3(nblocks(N3, 256)), dim3(256), 0, cl->stream, t, N3,
+                           (float)(1. / cl->s_out[0]), (float)(1. / cl->s_out[1]));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(cl->stream));
+        cl->scaled = true;
+    }
+    /* advance the fine lattice; the coarse one takes its restriction */
+    if (fine_steps > 0) {
+        CHK(gabe_step(fine, fine_steps));
+        CHK(gabe_restrict(fine, coarse));
+    }
+    return (int)cl->xs.size() - 1;
+}
+
+int gabe_closure_num_samples(gabe_closure *cl)
+{
+    if (!cl) return fail(GABE_ERR_ARG, "NULL closure");
+    return (int)cl->xs.size();
+}
+
+int gabe_closure_train(gabe_closure *cl, int iters, double lr, unsigned seed, double *loss_out)
+{
+    if (!cl || iters < 0 || !(lr > 0.)) return fail(GABE_ERR_ARG, "bad argument");
+    const int ns = (int)cl->xs.size();
+    if (ns == 0) return fail(GABE_ERR_STATE, "no samples");
+    std::vector<int> order(ns);
+    unsigned long long st = 0x2545F4914F6CDD1DULL ^ seed;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int it = 0; it < iters; it++) {
+        if (it % ns == 0) {   /* a new shuffle every pass over the samples */
+            for (int q = 0; q < ns; q++) order[q] = q;
+            for (int q = ns - 1; q > 0; q--) {
+                st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+                const int p = (int)((st >> 33) % (unsigned long long)(q + 1));
+                std::swap(order[q], order[p]);
+            }
+        }
+        const int smp = order[it % ns];
+        CHK(forward_loss(cl, cl->xs[smp], cl->ts[smp]));
+        CHK(backward(cl, cl->xs[smp]));
+        cl->adam_t++;
+        const float c1 = 1.f - std::pow(b1, (float)cl->adam_t), c2 = 1.f - std::pow(b2, (float)cl->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(nblocks(P_TOTAL, 256)), dim3(256), 0, cl->stream, cl->P, cl->G, cl->m, cl->v,
+                           P_TOTAL, (float)lr, b1, b2, eps, c1, c2);
+        HIPCHK(hipGetLastError());
+        CHK(repack(cl));
+        if (loss_out) CHK(read_loss(cl, &loss_out[it]));
+    }
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    return GABE_OK;
+}
+
+int gabe_closure_loss(gabe_closure *cl, int sample, double *loss)
+{
+    if (!cl || !loss) return fail(GABE_ERR_ARG, "NULL argument");
+    if (sample < 0 || sample >= (int)cl->xs.size()) return fail(GABE_ERR_ARG, "no sample %d", sample);
+    CHK(forward_loss(cl, cl->xs[sample], cl->ts[sample]));
+    return read_loss(cl, loss);
+}
+
+int gabe_closure_forward(gabe_closure *cl, const float *x, float *y)
+{
+    if (!cl || !x || !y) return fail(GABE_ERR_ARG, "NULL argument");
+    const long N3 = cl->N3;
+    std::vector<bf16> xb((size_t)4 * N3);
+    for (long q = 0; q < 4 * N3; q++) xb[q] = (bf16)x[q];
+    HIPCHK(hipMemcpy(cl->xbuf, xb.data(), sizeof(bf16) * 4 * N3, hipMemcpyHostToDevice));
+    ConvArgs a = conv_base(cl->N, cl->xbuf, cl->pk + PK_F1, cl->bias);
+    a.a_out = cl->a1;
+    CHK((launch_conv<4, EPI_HIDDEN>(a, cl->stream)));
+    a = conv_base(cl->N, cl->a1, cl->pk + PK_F2, cl->bias + 16);
+    a.a_out = cl->a2;
+    CHK((launch_conv<16, EPI_HIDDEN>(a, cl->stream)));
+    a = conv_base(cl->N, cl->a2, cl->pk + PK_F3, cl->bias + 32);
+    a.y_out = cl->ybuf;
+    CHK((launch_conv<16, EPI_OUT>(a, cl->stream)));
+    hipLaunchKernelGGL(k_lap_out, dim3(nblocks(N3, 256)), dim3(256), 0, cl->stream, cl->ybuf, cl->N, cl->wbuf);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_lap_out, dim3(nblocks(N3, 256)), dim3(256), 0, cl->stream, cl->wbuf, cl->N, cl->gybuf);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, cl->gybuf, sizeof(float) * 2 * N3, hipMemcpyDeviceToHost, cl->stream));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    return GABE_OK;
+}
+
+int gabe_closure_gradient(gabe_closure *cl, const float *x, const float *t, float *grad, double *loss)
+{
+    if (!cl || !x || !t || !grad || !loss) return fail(GABE_ERR_ARG, "NULL argument");
+    const long N3 = cl->N3;
+    std::vector<bf16> xb((size_t)4 * N3);
+    for (long q = 0; q < 4 * N3; q++) xb[q] = (bf16)x[q];
+    HIPCHK(hipMemcpy(cl->xbuf, xb.data(), sizeof(bf16) * 4 * N3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(cl->tbuf, t, sizeof(float) * 2 * N3, hipMemcpyHostToDevice));
+    CHK(forward_loss(cl, cl->xbuf, cl->tbuf));
+    CHK(backward(cl, cl->xbuf));
+    CHK(read_loss(cl, loss));
+    HIPCHK(hipMemcpy(grad, cl->G, sizeof(float) * P_TOTAL, hipMemcpyDeviceToHost));
+    return GABE_OK;
+}
+
+int gabe_closure_attach(gabe_ctx *c, gabe_closure *cl, double residual_threshold)
+{
+    if (!c || !cl) return fail(GABE_ERR_ARG, "NULL argument");
+    if (c->N != cl->N || !fp32_pair(c)) return fail(GABE_ERR_ARG, "the closure needs an FP32 two-field lattice of its N");
+    if (!c->own_stream) return fail(GABE_ERR_ARG, "ensemble members step in batches: attach to a single lattice");
+    if (!cl->scaled) return fail(GABE_ERR_STATE, "closure has no input/output scales (no samples, no set_scales)");
+    if (!(residual_threshold > 0.)) return fail(GABE_ERR_ARG, "residual threshold must be positive");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(cl->stream));
+    if (c->closure) closure_release(c);
+    ClosureRun *R = new ClosureRun;
+    c->closure = R;
+    R->N = cl->N;
+    R->thr = residual_threshold;
+    for (int q = 0; q < 4; q++) R->inv_s[q] = (float)(1. / cl->s_in[q]);
+    for (int q = 0; q < 2; q++) R->dt_s[q] = (float)(c->p.dt * cl->s_out[q]);
+    const long N3 = c->N3;
+    int rc = GABE_OK;
+    if (hipMalloc(&R->pk, sizeof(bf16x8) * PK_TOTAL) != hipSuccess || hipMalloc(&R->bias, sizeof(float) * 48) ||
+        hipMalloc(&R->x, sizeof(bf16) * 4 * N3) || hipMalloc(&R->a1, sizeof(bf16) * 16 * N3) ||
+        hipMalloc(&R->a2, sizeof(bf16) * 16 * N3) || hipMalloc(&R->u, sizeof(float) * 2 * N3) ||
+        hipMalloc(&R->w, sizeof(float) * 2 * N3) ||
+        hipMalloc(&R->gate, sizeof(int)) ||
+        hipMalloc(&R->fallback, sizeof(long long)) || hipMalloc(&R->rmax, sizeof(double)))
+        rc = fail(GABE_ERR_HIP, "closure attachment allocation failed");
+    const int one = 1;
+    const long long none = -1;
+    const double zero = 0.;
+    if (rc == GABE_OK &&
+        (hipMemcpy(R->pk, cl->pk, sizeof(bf16x8) * PK_TOTAL, hipMemcpyDeviceToDevice) != hipSuccess ||
+         hipMemcpy(R->bias, cl->bias, sizeof(float) * 48, hipMemcpyDeviceToDevice) != hipSuccess ||
+         hipMemcpy(R->gate, &one, sizeof one, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(R->fallback, &none, sizeof none, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(R->rmax, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(GABE_ERR_HIP, "closure attachment copy failed");
+    if (rc != GABE_OK) closure_release(c);
+    return rc;
+}
+
+int gabe_closure_detach(gabe_ctx *c)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    closure_release(c);
+    return GABE_OK;
+}
+
+int gabe_closure_status(gabe_ctx *c, int *active, long long *fallback_step, double *max_residual)
+{
+    if (!c) return fail(GABE_ERR_ARG, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int a = 0;
+    long long fb = -1;
+    double rm = 0.;
+    if (c->closure) {
+        HIPCHK(hipMemcpy(&a, c->closure->gate, sizeof a, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&fb, c->closure->fallback, sizeof fb, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&rm, c->closure->rmax, sizeof rm, hipMemcpyDeviceToHost));
+    }
+    if (active) *active = a;
+    if (fallback_step) *fallback_step = fb;
+    if (max_residual) *max_residual = rm;
+    return GABE_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,105 @@
+/*
+ * gabe_ctx.h -- the device context behind gabe_ctx and the helpers shared by
+ * the translation units that implement the C ABI (internal, not installed).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <string>
+
+#include "gabe.h"
+#include "gabe_internal.h"
+
+namespace gabe_detail {
+
+/* record the message of a failure for gabe_last_error() and return code */
+int fail(int code, const char *fmt, ...);
+/* Neumaier-compensated sum in index order */
+double csum(const double *x, long n);
+
+} // namespace gabe_detail
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return gabe_detail::fail(GABE_ERR_HIP, "%s failed: %s (%s:%d)", #expr,          \
+                                     hipGetErrorString(e_), __FILE__, __LINE__);            \
+    } while (0)
+
+#define FFTCHK(expr)                                                                        \
+    do {                                                                                    \
+        rocfft_status s_ = (expr);                                                          \
+        if (s_ != rocfft_status_success)                                                    \
+            return gabe_detail::fail(GABE_ERR_FFT, "%s failed: rocfft status %d", #expr, (int)s_); \
+    } while (0)
+
+#define CHK(expr)                        \
+    do {                                 \
+        int r_ = (expr);                 \
+        if (r_ != GABE_OK) return r_;    \
+    } while (0)
+
+struct ClosureRun;   /* a learned closure attached to a context (gabe_closure.hip) */
+
+struct FftPlan {
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    size_t work = 0;
+};
+
+struct gabe_ctx {
+    gabe_params p{};
+    int N = 0, nf = 0;
+    long N2 = 0, N3 = 0, M = 0;
+    double dx = 0.;
+    bool fp32 = false, gw = false, alloc_gw = false;
+    size_t esz = 8;              /* bytes per stored real */
+    hipStream_t stream = nullptr;
+    bool own_stream = true;          /* false for ensemble members (the ensemble's stream) */
+    hipStream_t side = nullptr;      /* sweep B runs here, beside the stress transform and GW chain */
+    hipEvent_t ev_fin = nullptr, ev_side = nullptr;
+    void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
+    void *vbuf[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *stress = nullptr;      /* [ns_arrays][N3] real */
+    void *shat = nullptr;        /* [ns_arrays][M] complex */
+    void *h = nullptr, *l = nullptr; /* [6][M] complex */
+    double *partials = nullptr;
+    int nblk = 0, ci = 0;
+    dim3 grid;
+    DevScalars *sc = nullptr;
+    StepScalars *ss = nullptr;
+    double *hist = nullptr;
+    int hist_cap = 0;
+    double *spec_out = nullptr;
+    FftPlan fft_stress, fft_spec, fft_c2r;
+    bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
+    bool gwfft = false;          /* i columns fused with the tensor-mode update */
+    long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
+    int fft_cw = 16;             /* column tile width of the j/i transforms */
+    void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */
+    void *fft_work = nullptr;
+    size_t fft_work_size = 0;
+    gabe_model model{};
+    /* spectral restriction onto a coarser context (gabe_restrict) */
+    FftPlan fft_rx;                  /* R2C of one array (fine side) or C2R (coarse side) */
+    void *rx_buf = nullptr;          /* one half-spectrum, complex storage type */
+    /* learned closure applied after every step (gabe_closure_attach) */
+    ClosureRun *closure = nullptr;
+};
+
+namespace gabe_detail {
+
+int make_plan(gabe_ctx *c, FftPlan &P, rocfft_transform_type type, bool dbl, size_t batch, int rank = 3,
+              size_t out_distance = 0);
+int run_plan(gabe_ctx *c, FftPlan &P, void *in, void *out);
+void destroy_plan(FftPlan &P);
+int read_scalars(gabe_ctx *c, DevScalars &d);
+int write_scalars(gabe_ctx *c, const DevScalars &d);
+/* closure kernels after a step (gabe_closure.hip) and the attachment's teardown */
+int closure_after_step(gabe_ctx *c);
+void closure_release(gabe_ctx *c);
+
+} // namespace gabe_detail

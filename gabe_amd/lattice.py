@@ -33,6 +33,10 @@ EXPORTS = (
     "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep",
     "gabe_ensemble_create", "gabe_ensemble_member", "gabe_ensemble_step", "gabe_ensemble_sync",
     "gabe_ensemble_destroy",
+    "gabe_restrict", "gabe_closure_create", "gabe_closure_destroy", "gabe_closure_params",
+    "gabe_closure_set_params", "gabe_closure_scales", "gabe_closure_set_scales", "gabe_closure_sample",
+    "gabe_closure_num_samples", "gabe_closure_train", "gabe_closure_loss", "gabe_closure_forward",
+    "gabe_closure_gradient", "gabe_closure_attach", "gabe_closure_detach", "gabe_closure_status",
 )
 
 
@@ -71,6 +75,7 @@ def load():
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
     L = C.CDLL(LIB_PATH)
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    fp = C.POINTER(C.c_float)
     sig = {
         "gabe_create": [C.POINTER(GabeParams), C.POINTER(vp)],
         "gabe_destroy": [vp],
@@ -102,6 +107,22 @@ def load():
         "gabe_ensemble_step": [vp, ip],
         "gabe_ensemble_sync": [vp],
         "gabe_ensemble_destroy": [vp],
+        "gabe_restrict": [vp, vp],
+        "gabe_closure_create": [ip, C.c_double, C.c_uint, C.POINTER(vp)],
+        "gabe_closure_destroy": [vp],
+        "gabe_closure_params": [vp, fp],
+        "gabe_closure_set_params": [vp, fp],
+        "gabe_closure_scales": [vp, dp, dp],
+        "gabe_closure_set_scales": [vp, dp, dp],
+        "gabe_closure_sample": [vp, vp, vp, ip, ip],
+        "gabe_closure_num_samples": [vp],
+        "gabe_closure_train": [vp, ip, C.c_double, C.c_uint, dp],
+        "gabe_closure_loss": [vp, ip, dp],
+        "gabe_closure_forward": [vp, fp, fp],
+        "gabe_closure_gradient": [vp, fp, fp, fp, dp],
+        "gabe_closure_attach": [vp, vp, C.c_double],
+        "gabe_closure_detach": [vp],
+        "gabe_closure_status": [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

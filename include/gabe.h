@@ -199,6 +199,82 @@ int gabe_ensemble_step(gabe_ensemble *ens, int nsteps);
 int gabe_ensemble_sync(gabe_ensemble *ens);
 int gabe_ensemble_destroy(gabe_ensemble *ens);
 
+/* ------------------------------------------------------------------------
+ * Learned closure of a coarse lattice (no counterpart in the reference).
+ *
+ * A 3-layer 3D convolutional network (3^3 periodic kernels, 4 -> 16 -> 16 -> 2
+ * channels, SiLU), BF16 weights and activations with FP32 accumulation on the
+ * matrix cores.  Input per site: the fluctuations (dphi, dchi, dphidot,
+ * dchidot), each divided by its input scale; the network's 2-channel output
+ * u enters through its lattice bi-Laplacian y = lap lap u (lap u = sum of
+ * the 6 neighbours minus 6 u), the form of the Laplacian's own truncation
+ * error, so the correction falls like (k dx)^4 at long wavelengths.  Attached to a lattice it adds
+ * dt * s_out[c] * y_c / a^2 to the velocity fluctuation of field c after
+ * every RK2 step.  Trained (Adam) on pairs made
+ * by the plain method: the state of a fine lattice spectrally restricted onto
+ * the coarse one, and as target what the coarse lattice's Laplacian misses of
+ * the fine one, R(lap_fine f) - lap_coarse(R f) with lap the 7-point
+ * Laplacian over dx^2 and R the restriction.  Lattices with a closure: FP32,
+ * two fields, N in {32, 64, 96, 128}.
+ *
+ * Parameters, FP32, GABE_CLOSURE_NPARAMS of them in this order:
+ *   W1[16][4][27], b1[16], W2[16][16][27], b2[16], W3[2][16][27], b3[2]
+ * with tap t = 9 (dz+1) + 3 (dy+1) + (dx+1), dz along the slowest index.
+ * Loss: sum over sites s and channels c of (y - t)^2 / (2 N^3).  (b3 drops
+ * out of y: its gradient is zero.)
+ * ---------------------------------------------------------------------- */
+#define GABE_CLOSURE_NPARAMS 9538
+typedef struct gabe_closure gabe_closure;
+
+/* Spectral restriction: the coarse context's state becomes the fine one's
+ * Fourier modes with |n_a| < N_coarse/2 (the coarse Nyquist planes zeroed),
+ * with the fine context's a, adot, t and FP64 means.  Same nflds, precision
+ * and L; the coarse N even and not larger. */
+int gabe_restrict(gabe_ctx *fine, gabe_ctx *coarse);
+
+/* A network for lattices of size N stepped with dt, weights drawn from seed. */
+int gabe_closure_create(int N, double dt, unsigned seed, gabe_closure **out);
+int gabe_closure_destroy(gabe_closure *cl);
+int gabe_closure_params(gabe_closure *cl, float *out);
+int gabe_closure_set_params(gabe_closure *cl, const float *in);   /* also resets Adam */
+/* Input scales s_in[4] and output scales s_out[2]: set by the first sample
+ * (rms of the inputs and of the raw targets) or by gabe_closure_set_scales. */
+int gabe_closure_scales(gabe_closure *cl, double *s_in, double *s_out);
+int gabe_closure_set_scales(gabe_closure *cl, const double *s_in, const double *s_out);
+
+/* One training pair from the plain method.  With restrict_first the coarse
+ * context first takes the restriction of the fine one; the input is the
+ * coarse state, the target R(lap_fine f) - lap_coarse(R f) per field.  Then
+ * the fine context takes fine_steps plain steps (0: none) and the coarse
+ * context the restriction of the advanced state, ready for the next call with
+ * restrict_first = 0.  Returns the sample index. */
+int gabe_closure_sample(gabe_closure *cl, gabe_ctx *fine, gabe_ctx *coarse, int fine_steps, int restrict_first);
+int gabe_closure_num_samples(gabe_closure *cl);
+
+/* iters Adam steps (learning rate lr), one whole sample per step, samples in
+ * a shuffled order drawn from seed; loss_out[iters] (optional) gets the loss
+ * of each step before its update. */
+int gabe_closure_train(gabe_closure *cl, int iters, double lr, unsigned seed, double *loss_out);
+/* loss on a stored sample */
+int gabe_closure_loss(gabe_closure *cl, int sample, double *loss);
+/* Network on host data in scaled units: x[N^3][4] -> y = lap lap u, [N^3][2]. */
+int gabe_closure_forward(gabe_closure *cl, const float *x, float *y);
+/* Loss and its gradient with respect to the parameters for host data
+ * x[N^3][4] and targets t[2][N^3] (the kernels training uses). */
+int gabe_closure_gradient(gabe_closure *cl, const float *x, const float *t, float *grad, double *loss);
+
+/* Apply the closure after every step of ctx (weights and scales are copied:
+ * cl may be destroyed afterwards).  Fallback: after each step the Friedmann
+ * residual R = |rho_dot + 3H(rho + p)| / (3 H rho) of the per-step history
+ * (central difference, one step back) is checked on the device; once R
+ * exceeds residual_threshold (or is not finite) the closure stays off and the
+ * lattice continues with the plain step. */
+int gabe_closure_attach(gabe_ctx *ctx, gabe_closure *cl, double residual_threshold);
+int gabe_closure_detach(gabe_ctx *ctx);
+/* active: 1 while the closure is applied; fallback_step: step count when it
+ * switched off (-1 if never); max_residual: largest R checked so far. */
+int gabe_closure_status(gabe_ctx *ctx, int *active, long long *fallback_step, double *max_residual);
+
 #ifdef __cplusplus
 }
 #endif
