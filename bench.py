@@ -71,6 +71,9 @@ def read_traffic(kernel, N):
 
 
 def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
+    """Time `steps` steps after `warmup`; then `profile_steps` more with
+    per-kernel events.  Returns the time, kernel means and the final state,
+    scalars, spectra and Friedmann residual for the precision comparison."""
     from gabe_amd import host
     lat = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb)
     lat.step(warmup)
@@ -80,10 +83,47 @@ def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
     lat.sync()
     t1 = time.perf_counter()
     ktimes = lat.kernel_times(profile_steps) if profile_steps else {}
-    sc = lat.get_scalars()
-    state = lat.download_state() if precision == 1 else None
+    final = {"scalars": lat.get_scalars(), "state": lat.download_state(), "spectra": lat.specOut(),
+             "friedmann_max": host.friedmann_residual(lat.history())[0]}
     lat.close()
-    return t1 - t0, ktimes, sc, state
+    return t1 - t0, ktimes, final
+
+
+def precision_report(f32, f64):
+    """FP32 form against the FP64 form after the same steps from the same
+    initial state: field L2 error relative to the fluctuation, per-bin
+    spectra (bins above 1e-9 of the column maximum), energy and the Friedmann
+    residual of each."""
+    import numpy as np
+    out = {}
+    for name, k in (("field", 0), ("velocity", 1)):
+        a, b = f32["state"][k], f64["state"][k]
+        errs = []
+        for f in range(b.shape[0]):
+            fl = b[f] - b[f].mean()
+            errs.append(float(np.linalg.norm(a[f] - b[f]) / np.linalg.norm(fl)))
+        out[f"{name}_l2_rel"] = errs
+    sp32, sp64 = f32["spectra"], f64["spectra"]
+    cols = {"field_spectra": [0, 2], "gw_spectrum": [4]}
+    for name, cs in cols.items():
+        worst = 0.0
+        for c in cs:
+            ref, got = sp64[c], sp32[c]
+            if not (np.all(np.isfinite(ref)) and np.all(np.isfinite(got))):
+                out[f"{name}_nonfinite_column"] = c
+                worst = float("nan")
+                continue
+            live = np.abs(ref) > 1e-9 * np.abs(ref).max()
+            live[0] = False
+            if live.any():
+                worst = max(worst, float(np.max(np.abs(got[live] / ref[live] - 1.0))))
+        out[f"{name}_max_rel"] = worst
+    r32, r64 = f32["scalars"]["edrho"][0], f64["scalars"]["edrho"][0]
+    out["edrho_rel"] = abs(r32 - r64) / abs(r64)
+    out["a_rel"] = abs(f32["scalars"]["a"][0] - f64["scalars"]["a"][0]) / f64["scalars"]["a"][0]
+    out["friedmann_max_fp32"] = float(f32["friedmann_max"])
+    out["friedmann_max_fp64"] = float(f64["friedmann_max"])
+    return out
 
 
 def ensemble_members(sizes=(32, 64, 96, 128), per_size=16, seed=20240):
@@ -158,6 +198,36 @@ def run_ensemble(steps, warmup):
     }
 
 
+def run_closure(steps, warmup):
+    """128^3 with tensor modes: the plain step against the step with the
+    committed trained closure (gabe_amd/closure_128.npz) attached."""
+    from gabe_amd import closure as cm
+    from gabe_amd import configs, host
+    from gabe_amd.lattice import GABE_FP32
+    path = os.path.join(ROOT, "gabe_amd", "closure_128.npz")
+    cfg = configs.lambda_phi4_config(N=128, calc_gws=1)
+    cl = cm.Closure.from_file(path)
+    out = {}
+    for name in ("plain", "closure"):
+        lat = host.initialized_lattice(cfg, precision=GABE_FP32)
+        if name == "closure":
+            cm.attach(lat, cl, 1e-3)
+        lat.step(warmup)
+        lat.sync()
+        t0 = time.perf_counter()
+        lat.step(steps)
+        lat.sync()
+        out[f"{name}_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
+        if name == "closure":
+            out["closure_active"] = cm.status(lat)[0]
+        lat.close()
+    cl.close()
+    out["cost_vs_plain"] = out["closure_ms_per_step"] / out["plain_ms_per_step"]
+    out["workload"] = "128^3, 2 fields + GW, FP32 form, trained closure (3x 3^3 conv, 16 ch, BF16) after every step"
+    out["steps"] = steps
+    return out
+
+
 def cpu_baseline(cfg, state, nsteps=2):
     """The CPU FP64 oracle, OpenMP over the outer index, on this host's cores."""
     from oracle import pyoracle as po
@@ -194,6 +264,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-lattice ensemble (0: skip)")
+    ap.add_argument("--closure-steps", type=int, default=200, help="steps of the 128^3 closure timing (0: skip)")
     ap.add_argument("--ppb", type=int, default=0, help="lattice planes per sweep workgroup (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -209,15 +280,19 @@ def main():
     sites = N ** 3
 
     print(f"[bench] FP32 {N}^3: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-    dt32, k32, sc32, state = run_device(cfg, GABE_FP32, args.steps, args.warmup, args.profile_steps, args.ppb)
+    dt32, k32, fin32 = run_device(cfg, GABE_FP32, args.steps, args.warmup, args.profile_steps, args.ppb)
+    sc32, state = fin32["scalars"], fin32["state"]
     rate32 = sites * args.steps / dt32
     fp64 = None
     if not args.no_fp64:
         print(f"[bench] FP64 {N}^3", file=sys.stderr, flush=True)
-        dt64, k64, sc64, _ = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps, args.ppb)
+        dt64, k64, fin64 = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps, args.ppb)
         fp64 = {"value": sites * args.steps / dt64, "ms_per_step": 1e3 * dt64 / args.steps,
                 "kernels_ms": k64,
-                "step_hbm_frac": b_alg(8, N) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9)}
+                "step_hbm_frac": b_alg(8, N) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9),
+                "fp32_vs_fp64": dict(steps=args.warmup + args.steps + args.profile_steps,
+                                     **precision_report(fin32, fin64))}
+        del fin64
 
     ab = alg_bytes_per_site(4, N)
     roofline = None
@@ -235,6 +310,12 @@ def main():
     if args.ensemble_steps:
         print(f"[bench] ensemble of 64 lattices, {args.ensemble_steps} steps", file=sys.stderr, flush=True)
         ensemble = run_ensemble(args.ensemble_steps, min(args.warmup, 20))
+
+    del fin32
+    closure = None
+    if args.closure_steps:
+        print(f"[bench] closure at 128^3, {args.closure_steps} steps", file=sys.stderr, flush=True)
+        closure = run_closure(args.closure_steps, min(args.warmup, 20))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -264,6 +345,7 @@ def main():
         "kernels": kernels,
         "fp64": fp64,
         "ensemble": ensemble,
+        "closure": closure,
         "cpu_baseline": cpu,
         "final_a": sc32["a"][0],
     }
