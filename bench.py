@@ -25,22 +25,24 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "lattice site-updates/s (RK2, 2 fields+GW) at 256³ FP32 vs FP64; % HBM peak"
 
 
-def alg_bytes_per_site(s, N):
+def alg_bytes_per_site(s, N, rows_in_sweep=True):
     """Compulsory HBM bytes per site-update of each kernel class of this
     pipeline (s = bytes per real, r = (N/2+1)/N modes per site).
 
-    sweep_a    reads the 4 state arrays, writes the 12 real stress arrays
-    stress_fft k-row R2C (reads 12 reals, writes 12 half-spectra) + j columns
-               (reads and writes them)
+    sweep_a    reads the 4 state arrays; with the k-rows transformed in the
+               sweep (FP32, N 16..256) writes the 12 half-spectrum rows,
+               otherwise the 12 real stress arrays
+    stress_fft j columns in place (read + write the 12 half-spectra), plus
+               the library k-row R2C when the sweep does not transform rows
     gw_modes   i columns fused with the mode update: reads the 12 half-spectra
                and h, l (two TT polarisations, complex), writes h, l
     sweep_b    reads and writes the 4 state arrays
     """
     r = (N // 2 + 1) / N
     return {
-        "sweep_a": 16 * s,
+        "sweep_a": 4 * s + (24 * s * r if rows_in_sweep else 12 * s),
         "finalize": 0.0,
-        "stress_fft": 12 * s + 24 * s * r + 48 * s * r,
+        "stress_fft": 48 * s * r + (0 if rows_in_sweep else 12 * s + 24 * s * r),
         "gw_modes": 24 * s * r + 16 * s * r,
         "sweep_b": 8 * s,
     }
@@ -294,7 +296,7 @@ def main():
                                      **precision_report(fin32, fin64))}
         del fin64
 
-    ab = alg_bytes_per_site(4, N)
+    ab = alg_bytes_per_site(4, N, rows_in_sweep=16 <= N <= 256 and (N & (N - 1)) == 0)
     roofline = None
     if k32:
         dom = max((k for k in k32 if k != "finalize"), key=lambda k: k32[k])

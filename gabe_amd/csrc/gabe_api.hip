@@ -127,6 +127,10 @@ SweepArgs<T> sweep_args(gabe_ctx *c)
         a.vo[f] = (T *)c->vbuf[1 - c->cur] + (long)f * c->N3;
     }
     a.stress = (T *)c->stress;
+    a.shat = (T *)c->shat;
+    a.tw = (const T *)c->twiddle;
+    a.kpad = c->kpad;
+    a.rowfft = c->rowfft ? c->N : 0;
     a.partials = c->partials;
     a.sc = c->sc;
     a.ss = c->ss;
@@ -192,7 +196,7 @@ int launch_gw_ctx(gabe_ctx *c)
 int stress_fft(gabe_ctx *c, bool with_i = true)
 {
     if (!c->colfft) return run_plan(c, c->fft_stress, c->stress, c->shat);
-    CHK(run_plan(c, c->fft_stress, c->stress, c->shat));            /* rows along k */
+    if (!c->rowfft) CHK(run_plan(c, c->fft_stress, c->stress, c->shat));   /* rows along k */
     const long N = c->N, KP = N / 2 + 1, kp = c->kpad;
     ColFftView vj{c->shat, kp, N * kp, 0, (int)KP, 1, 12 * N};        /* along j */
     ColFftView vi{c->shat, N * kp, N * N * kp, kp, (int)KP, (int)N, 12}; /* along i */
@@ -352,6 +356,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->colfft = c->alloc_gw && gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
     c->kpad = c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1;
     c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
+    c->rowfft = c->gwfft && c->N >= 16 && c->N <= GABE_BLOCK && !std::getenv("GABE_LIBRARY_ROWS");
     if (const char *e = std::getenv("GABE_FFT_TILE")) c->fft_cw = std::atoi(e) == 32 ? 32 : 16;
     HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
     HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad)));

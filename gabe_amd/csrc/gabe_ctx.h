@@ -77,6 +77,7 @@ struct gabe_ctx {
     FftPlan fft_stress, fft_spec, fft_c2r;
     bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
     bool gwfft = false;          /* i columns fused with the tensor-mode update */
+    bool rowfft = false;         /* k rows transformed inside sweep A */
     long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
     int fft_cw = 16;             /* column tile width of the j/i transforms */
     void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */

@@ -31,11 +31,13 @@ constexpr int load_batch(int n)
     return 1;
 }
 
-/* One RK2 stage: components 6 ST .. 6 ST + 5 of the source: load the six
- * N x CW tiles, transform along i (two steps, twiddles read from the first
- * tile's padding column), return to natural order, and project the owned
- * modes on e+ and ex into acc[mode][pol]. */
-template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST>
+/* One round of an RK2 stage: components 6 ST + C0 .. 6 ST + C0 + CPR - 1 of
+ * the source (CPR = 6: the whole stage; CPR = 3: half of it, the projection
+ * being linear in the components): load the CPR N x CW tiles, transform along
+ * i (two steps, twiddles read from the first tile's padding column), return
+ * to natural order, and project the owned modes on e+ and ex into
+ * acc[mode][pol]. */
+template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0>
 __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CPR][N1 * N2][CW + 1],
                                              const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
                                              const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
@@ -49,10 +51,10 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
     constexpr int LB = load_batch(LPT);
     constexpr int S1T = (CPR * N2 * CW + NT - 1) / NT;
     constexpr int S2T = (CPR * N1 * CW + NT - 1) / NT;
-    static_assert(CPR == 6, "one round transforms the six components of a stage");
+    static_assert((CPR == 6 && C0 == 0) || (CPR == 3 && (C0 == 0 || C0 == 3)), "rounds cover a stage");
     static_assert(LPT % LB == 0, "loads must split into whole batches");
     {
-        constexpr int c0 = 0;
+        constexpr int c0 = C0;
 #pragma unroll
         for (int t0 = 0; t0 < LPT; t0 += LB) {
             C v[LB];
@@ -183,8 +185,10 @@ __global__ __launch_bounds__(NT) void k_gw_ifft(GwFftArgs<T> A)
 #pragma unroll
         for (int p = 0; p < 2; p++) { acc0[u][p].x = acc0[u][p].y = (T)0; acc1[u][p].x = acc1[u][p].y = (T)0; }
     }
-    stage_rounds<T, N1, N2, CW, CPR, NT, 0>(tile, Sh, SM, rstride, width, bu, bv, acc0);
-    stage_rounds<T, N1, N2, CW, CPR, NT, 1>(tile, Sh, SM, rstride, width, bu, bv, acc1);
+    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0>(tile, Sh, SM, rstride, width, bu, bv, acc0);
+    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3>(tile, Sh, SM, rstride, width, bu, bv, acc0);
+    stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(tile, Sh, SM, rstride, width, bu, bv, acc1);
+    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3>(tile, Sh, SM, rstride, width, bu, bv, acc1);
 
     /* both midpoint-RK2 stages of the owned modes (derivs, g2evolutionpert.cpp:486-501) */
     const StepScalars *ss = A.ss;
@@ -242,13 +246,18 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
     case 64: launch_cfg<float, 8, 8, 8, 6, 512>(a, s); break;
     case 128: launch_cfg<float, 16, 8, 8, 6, 1024>(a, s); break;
     case 256: {
-        /* tile variants (kz columns per workgroup, threads): see the tuning notes */
+        /* tile variants (kz columns per workgroup, components per round,
+         * threads); the default, 4 columns and 3 components per round on 256
+         * threads, fits three workgroups per CU and measured fastest */
         const char *v = std::getenv("GABE_GWFFT_TILE");
         const int t = v ? std::atoi(v) : 0;
         if (t == 1) launch_cfg<float, 16, 16, 4, 6, 512>(a, s);
         else if (t == 2) launch_cfg<float, 16, 16, 4, 6, 256>(a, s);
         else if (t == 3) launch_cfg<float, 16, 16, 8, 6, 512>(a, s);
-        else launch_cfg<float, 16, 16, 8, 6, 1024>(a, s);
+        else if (t == 4) launch_cfg<float, 16, 16, 8, 3, 512>(a, s);
+        else if (t == 5) launch_cfg<float, 16, 16, 4, 3, 512>(a, s);
+        else if (t == 6) launch_cfg<float, 16, 16, 8, 6, 1024>(a, s);
+        else launch_cfg<float, 16, 16, 4, 3, 256>(a, s);
         break;
     }
     default: break;

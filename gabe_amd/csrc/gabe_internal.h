@@ -54,6 +54,10 @@ struct SweepArgs {
     T *fo[GABE_MAXF];       /* sweep B output state */
     T *vo[GABE_MAXF];
     T *stress;              /* [12][N^3]: S(f0) comps 0-5, S(f1) comps 6-11 */
+    T *shat;                /* [12][N][N][kpad] complex: row transforms written by the fused sweep A */
+    const T *tw;            /* exp(-2 pi i e / N), e < N, complex */
+    long kpad;
+    int rowfft;             /* N when sweep A transforms its k-rows itself, else 0 */
     double *partials;       /* [nsums][nblk] */
     const DevScalars *sc;
     const StepScalars *ss;

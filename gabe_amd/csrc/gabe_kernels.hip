@@ -20,6 +20,9 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
+#include "gabe_fft_dev.h"
 #include "gabe_internal.h"
 #include "gabe_modes.h"
 
@@ -63,13 +66,68 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
 }
 
 /* ------------------------------------------------------------------ sweeps */
-/* Thread layout: one thread per (j,k) column of the N x N plane, marching
- * through ci consecutive i-planes; the i-1/i/i+1 values stay in registers,
- * the four lateral neighbours come through the L1/L2 caches (they are the
- * centre loads of the adjacent lanes and rows).  Consecutive lanes read
- * consecutive k: every load is coalesced. */
-template <typename T, int NF, int MODE, bool GW>
-__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy)
+/* Row R2C of the 12 stress components of the GABE_BLOCK / N k-rows one sweep
+ * workgroup holds, for npl (1 or 2) consecutive planes from plane i, written
+ * into the padded half-spectrum rows of shat (the library row transform it
+ * replaces).  Each stage's components arrive in LDS as three complex rows,
+ * z = S_c + i S_(c+1) for the pairs (xx,xy), (xz,yy), (yz,zz); one complex
+ * transform of length N per pair (two-step N = N1 N2, in-register DFTs) and
+ * the split
+ *   S_c^(m) = (Z(m) + conj Z(N-m)) / 2,  S_(c+1)^(m) = (Z(m) - conj Z(N-m)) / 2i
+ * give both real rows' transforms, m = 0..N/2.  Two planes per call halve
+ * the barriers per plane, which stall the sweep's loads. */
+template <int N>
+__device__ __forceinline__ void row_r2c(float2 *zl, float2 *zo, const SweepArgs<float> &A, int i, int j0, int npl)
+{
+    using gabe_fft::cmul;
+    constexpr int N1 = N == 16 ? 4 : (N <= 64 ? 8 : 16);
+    constexpr int N2 = N / N1;
+    constexpr int R = GABE_BLOCK / N, SPP = 6 * R, H = N / 2 + 1;   /* sequences per plane */
+    const int nseq = SPP * npl;
+    const float2 *tw = (const float2 *)A.tw;
+    for (int task = threadIdx.x; task < nseq * N2; task += GABE_BLOCK) {
+        const int sq = task / N2, n2 = task - sq * N2;
+        float2 *b = zl + sq * N;
+        float2 x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) x[n1] = b[N2 * n1 + n2];
+        gabe_fft::dft_reg<N1, float>(x);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+            float2 v = x[k1];
+            if (k1 != 0 && n2 != 0) v = cmul(v, tw[(n2 * k1) % N]);
+            b[N2 * k1 + n2] = v;
+        }
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < nseq * N1; task += GABE_BLOCK) {
+        const int sq = task / N1, k1 = task - sq * N1;
+        const float2 *b = zl + sq * N;
+        float2 *o = zo + sq * N;
+        float2 y[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; n2++) y[n2] = b[N2 * k1 + n2];
+        gabe_fft::dft_reg<N2, float>(y);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) o[k1 + N1 * k2] = y[k2];
+    }
+    __syncthreads();
+    const long comp = (long)N * N * A.kpad;
+    float2 *out = (float2 *)A.shat;
+    for (int task = threadIdx.x; task < nseq * H; task += GABE_BLOCK) {
+        const int sq = task / H, m = task - sq * H;
+        const int pl = sq / SPP, ps = sq - pl * SPP, pr = ps / R, r = ps - pr * R;
+        const float2 *o = zo + sq * N;
+        const float2 Z = o[m], W = o[m == 0 ? 0 : N - m];
+        const long row = ((long)(i + pl) * N + (j0 + r)) * A.kpad + m;
+        out[(2 * pr) * comp + row] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
+        out[(2 * pr + 1) * comp + row] = make_float2(0.5f * (Z.y + W.y), -0.5f * (Z.x - W.x));
+    }
+}
+
+template <typename T, int NF, int MODE, bool GW, int ROWN = 0>
+__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy, float2 *zl = nullptr,
+                                           float2 *zo = nullptr)
 {
     constexpr int NS = gabe_nsums(NF);
     const int N = A.N;
@@ -225,15 +283,30 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                             sxy += g[0] * g[1]; sxz += g[0] * g[2]; syz += g[1] * g[2];
                         }
                         const T q3 = (sxx + syy + szz) * (T)(1. / 3.);
-                        T *S = A.stress + (long)(6 * st) * A.N3 + base + pp;
-                        S[0 * A.N3] = (sxx - q3) * idx2;
-                        S[1 * A.N3] = sxy * idx2;
-                        S[2 * A.N3] = sxz * idx2;
-                        S[3 * A.N3] = (syy - q3) * idx2;
-                        S[4 * A.N3] = syz * idx2;
-                        S[5 * A.N3] = (szz - q3) * idx2;
+                        if constexpr (ROWN > 0) {
+                            float2 *z = zl + (((i - i0) & 1) * 6 + 3 * st) * GABE_BLOCK + threadIdx.x;
+                            z[0] = make_float2((sxx - q3) * idx2, sxy * idx2);
+                            z[GABE_BLOCK] = make_float2(sxz * idx2, (syy - q3) * idx2);
+                            z[2 * GABE_BLOCK] = make_float2(syz * idx2, (szz - q3) * idx2);
+                        } else {
+                            T *S = A.stress + (long)(6 * st) * A.N3 + base + pp;
+                            S[0 * A.N3] = (sxx - q3) * idx2;
+                            S[1 * A.N3] = sxy * idx2;
+                            S[2 * A.N3] = sxz * idx2;
+                            S[3 * A.N3] = (syy - q3) * idx2;
+                            S[4 * A.N3] = syz * idx2;
+                            S[5 * A.N3] = (szz - q3) * idx2;
+                        }
                     }
                 }
+            }
+        }
+        if constexpr (ROWN > 0) {
+            if (((i - i0) & 1) || i + 1 == i1) {
+                const int npl = ((i - i0) & 1) ? 2 : 1;
+                __syncthreads();
+                row_r2c<ROWN>(zl, zo, A, i + 1 - npl, bx * (GABE_BLOCK / ROWN), npl);
+                __syncthreads();
             }
         }
 #pragma unroll
@@ -243,6 +316,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
         }
     }
     if (MODE != SW_B) block_store_partials<NS>(acc, A.partials, A.nblk, bid);
+}
+
+/* sweep A with the stress rows transformed in the workgroup (FP32, N 16..256) */
+template <int NF, int ROWN>
+__global__ __launch_bounds__(GABE_BLOCK) void k_sweep_rowfft(SweepArgs<float> A)
+{
+    __shared__ float2 zl[2 * 6 * GABE_BLOCK], zo[2 * 6 * GABE_BLOCK];
+    sweep_body<float, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo);
 }
 
 template <typename T, int NF, int MODE, bool GW>
@@ -721,9 +802,25 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_add_real(T *dst, const double *a
 } // namespace
 
 /* ---------------------------------------------------------------- launchers */
+template <int NF>
+static bool launch_rowfft(const SweepArgs<float> &a, dim3 grid, hipStream_t s)
+{
+    switch (a.rowfft) {
+    case 16: hipLaunchKernelGGL((k_sweep_rowfft<NF, 16>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 32: hipLaunchKernelGGL((k_sweep_rowfft<NF, 32>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 64: hipLaunchKernelGGL((k_sweep_rowfft<NF, 64>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 128: hipLaunchKernelGGL((k_sweep_rowfft<NF, 128>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 256: hipLaunchKernelGGL((k_sweep_rowfft<NF, 256>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    default: return false;
+    }
+}
+
 template <typename T, int NF>
 static void launch_sweep_nf(int mode, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s)
 {
+    if constexpr (std::is_same<T, float>::value) {
+        if (mode == SW_A && gw && a.rowfft && launch_rowfft<NF>(a, grid, s)) return;
+    }
     if (mode == SW_ENERGY) hipLaunchKernelGGL((k_sweep<T, NF, SW_ENERGY, false>), grid, dim3(GABE_BLOCK), 0, s, a);
     else if (mode == SW_A && gw) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, true>), grid, dim3(GABE_BLOCK), 0, s, a);
     else if (mode == SW_A) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, false>), grid, dim3(GABE_BLOCK), 0, s, a);
