@@ -225,6 +225,13 @@ def run_closure(steps, warmup):
         lat.close()
     cl.close()
     out["cost_vs_plain"] = out["closure_ms_per_step"] / out["plain_ms_per_step"]
+    # the network's contraction on the matrix cores: 2 x (4*16 + 16*16 + 16*2) x 27 flop per site
+    flops = 2.0 * (4 * 16 + 16 * 16 + 16 * 2) * 27 * cfg.N ** 3
+    t = (out["closure_ms_per_step"] - out["plain_ms_per_step"]) * 1e-3
+    out["roofline"] = {"bound": "mfma", "dtype": "bf16 (fp32 accumulate)", "flop_per_eval": flops,
+                       "achieved": flops / t / 1e12 if t > 0 else None, "peak": 2500.0, "unit": "TFLOP/s",
+                       "frac": flops / t / 2.5e15 if t > 0 else None,
+                       "note": "time = closure step minus plain step (includes the input packing and the bi-Laplacian)"}
     out["workload"] = "128^3, 2 fields + GW, FP32 form, trained closure (3x 3^3 conv, 16 ch, BF16) after every step"
     out["steps"] = steps
     return out

@@ -218,11 +218,11 @@ int gw_modes(gabe_ctx *c)
     const double dx3 = c->dx * c->dx * c->dx;
     if (c->fp32) {
         GwFftArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, (const float *)c->twiddle,
-                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3};
+                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4};
         launch_gw_ifft<float>(g, c->stream);
     } else {
         GwFftArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss,
-                            (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3};
+                            (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4};
         launch_gw_ifft<double>(g, c->stream);
     }
     HIPCHK(hipGetLastError());
@@ -357,14 +357,19 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->kpad = c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1;
     c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
     c->rowfft = c->gwfft && c->N >= 16 && c->N <= GABE_BLOCK && !std::getenv("GABE_LIBRARY_ROWS");
+    c->lay.N = c->N;
+    c->lay.KP = c->N / 2 + 1;
+    c->lay.nt4 = (int)((c->lay.KP + 3) / 4);
+    c->lay.tiled = c->gwfft ? 1 : 0;
+    c->mslots = c->lay.tiled ? (long)c->N * c->lay.nt4 * c->N * 4 : c->M;
     if (const char *e = std::getenv("GABE_FFT_TILE")) c->fft_cw = std::atoi(e) == 32 ? 32 : 16;
     HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
     HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad)));
     if (c->alloc_gw) {
-        HIPCHK(hipMalloc(&c->h, 2 * c->esz * 2 * c->M));
-        HIPCHK(hipMalloc(&c->l, 2 * c->esz * 2 * c->M));
-        HIPCHK(hipMemset(c->h, 0, 2 * c->esz * 2 * c->M));
-        HIPCHK(hipMemset(c->l, 0, 2 * c->esz * 2 * c->M));
+        HIPCHK(hipMalloc(&c->h, 2 * c->esz * 2 * c->mslots));
+        HIPCHK(hipMalloc(&c->l, 2 * c->esz * 2 * c->mslots));
+        HIPCHK(hipMemset(c->h, 0, 2 * c->esz * 2 * c->mslots));
+        HIPCHK(hipMemset(c->l, 0, 2 * c->esz * 2 * c->mslots));
     }
 
     /* sweep tiling: one thread per plane column, ci planes per workgroup */
@@ -623,7 +628,7 @@ static CkptHeader ckpt_header(const gabe_ctx *c, int hist_rows)
     h.gw_spectrum_quirk = c->p.gw_spectrum_quirk;
     h.hist_rows = hist_rows;
     h.state_bytes = (unsigned long long)c->nf * c->N3 * c->esz;
-    h.gw_bytes = c->alloc_gw ? 2ULL * c->esz * 2 * c->M : 0;
+    h.gw_bytes = c->alloc_gw ? 2ULL * c->esz * 2 * c->mslots : 0;
     h.total = sizeof(CkptHeader) + sizeof(DevScalars) + sizeof(double) * GABE_HIST_COLS * (unsigned long long)hist_rows +
               2 * h.state_bytes + 2 * h.gw_bytes;
     return h;
@@ -733,8 +738,8 @@ int gabe_upload_gw(gabe_ctx *c, const double *h, const double *l)
             rc = fail(GABE_ERR_HIP, "upload of tensor modes failed");
             break;
         }
-        if (c->fp32) launch_gw_pack<float>(tmp, (float *)dst[a], c->N, M, c->stream);
-        else launch_gw_pack<double>(tmp, (double *)dst[a], c->N, M, c->stream);
+        if (c->fp32) launch_gw_pack<float>(tmp, (float *)dst[a], c->lay, M, c->stream);
+        else launch_gw_pack<double>(tmp, (double *)dst[a], c->lay, M, c->stream);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
             rc = fail(GABE_ERR_HIP, "tensor mode packing failed");
     }
@@ -755,8 +760,8 @@ int gabe_download_gw(gabe_ctx *c, double *h, double *l)
     int rc = GABE_OK;
     for (int a = 0; a < 2 && rc == GABE_OK; a++) {
         if (!dst[a]) continue;
-        if (c->fp32) launch_gw_unpack<float>((const float *)src[a], tmp, c->N, M, c->stream);
-        else launch_gw_unpack<double>((const double *)src[a], tmp, c->N, M, c->stream);
+        if (c->fp32) launch_gw_unpack<float>((const float *)src[a], tmp, c->lay, M, c->stream);
+        else launch_gw_unpack<double>((const double *)src[a], tmp, c->lay, M, c->stream);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
             hipMemcpy(dst[a], tmp, sizeof(double) * 12 * M, hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(GABE_ERR_HIP, "tensor mode download failed");
@@ -850,12 +855,14 @@ int gabe_spectra(gabe_ctx *c, double *out)
         a.l = (const float *)c->l;
         for (int f = 0; f < nf; f++) a.dc[f] = d.pbar[f] * (double)c->N3;
         a.nf = nf; a.N = N; a.numbins = nb; a.quirk = c->p.gw_spectrum_quirk; a.M = c->M; a.out = c->spec_out;
+        a.lay = c->lay;
         launch_spectra<float>(a, c->stream);
     } else {
         SpecArgs<double> a{};
         a.fhat = (const double *)c->shat;
         a.l = (const double *)c->l;
         a.nf = nf; a.N = N; a.numbins = nb; a.quirk = c->p.gw_spectrum_quirk; a.M = c->M; a.out = c->spec_out;
+        a.lay = c->lay;
         launch_spectra<double>(a, c->stream);
     }
     HIPCHK(hipGetLastError());

@@ -65,7 +65,9 @@ struct gabe_ctx {
     int cur = 0;
     void *stress = nullptr;      /* [ns_arrays][N3] real */
     void *shat = nullptr;        /* [ns_arrays][M] complex */
-    void *h = nullptr, *l = nullptr; /* [6][M] complex */
+    void *h = nullptr, *l = nullptr; /* mslots polarisation pairs (complex) each, order lay */
+    ModeLayout lay{};
+    long mslots = 0;
     double *partials = nullptr;
     int nblk = 0, ci = 0;
     dim3 grid;

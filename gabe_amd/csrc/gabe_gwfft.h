@@ -12,12 +12,13 @@
 template <typename T>
 struct GwFftArgs {
     const T *shat;      /* [12][N][N][kpad] complex, transformed along k and j */
-    T *h, *l;           /* [M][2] complex polarisation pairs */
+    T *h, *l;           /* polarisation pairs, tiled order (gabe_modes.h) */
     const StepScalars *ss;
     const T *tw;        /* exp(-2 pi i e / N), e < N, complex */
     int N;
     long M, kpad;
     double L, dt, dx3;
+    int nt4;            /* kz tiles of 4 of the tiled mode order */
 };
 
 /* whether the fused kernel exists for this size and storage type: FP32,

@@ -206,7 +206,8 @@ __global__ __launch_bounds__(NT) void k_gw_ifft(GwFftArgs<T> A)
         const int px = i <= N / 2 ? i : i - N, pz = q * CW + col;
         const double p2 = (double)(px * px + py * py + pz * pz);
         const T kk0 = (T)(k2u * p2 * ia0), kk1 = (T)(k2u * p2 * ia1);
-        const long m = ((long)i * N + j) * KP + pz;
+        /* tiled mode order [j][kz/4][i][kz%4] */
+        const long m = (((long)j * A.nt4 + (pz >> 2)) * N + i) * 4 + (pz & 3);
 #pragma unroll
         for (int pol = 0; pol < 2; pol++) {
             const C h0 = hh[2 * m + pol], l0 = ll[2 * m + pol];

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gabe_model.h"
+#include "gabe_modes.h"
 
 #define GABE_MAXF 2
 #define GABE_BLOCK 256
@@ -92,7 +93,8 @@ struct GwArgs {
 template <typename T>
 struct SpecArgs {
     const T *fhat;      /* [nf][M] complex field transforms */
-    const T *l;         /* [M][2] complex polarisation pairs, or null */
+    const T *l;         /* polarisation pairs in the order lay, or null */
+    ModeLayout lay;
     double dc[GABE_MAXF];   /* added to the real part of each field's k=0 mode */
     int nf, N, numbins, quirk;
     long M;
@@ -109,9 +111,9 @@ void launch_finalize_batch(const FinArgs *descs, int nf, int count, hipStream_t 
 template <typename T>
 void launch_gw(const GwArgs<T> &a, hipStream_t s);
 template <typename T>
-void launch_gw_pack(const double *ref, T *pol, int N, long M, hipStream_t s);
+void launch_gw_pack(const double *ref, T *pol, ModeLayout lay, long M, hipStream_t s);
 template <typename T>
-void launch_gw_unpack(const T *pol, double *ref, int N, long M, hipStream_t s);
+void launch_gw_unpack(const T *pol, double *ref, ModeLayout lay, long M, hipStream_t s);
 template <typename T>
 void launch_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3, double *out, hipStream_t s);
 template <typename T>

@@ -76,6 +76,15 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
  *   S_c^(m) = (Z(m) + conj Z(N-m)) / 2,  S_(c+1)^(m) = (Z(m) - conj Z(N-m)) / 2i
  * give both real rows' transforms, m = 0..N/2.  Two planes per call halve
  * the barriers per plane, which stall the sweep's loads. */
+/* LDS layout of the row sequences: element n of sequence sq at
+ * sq * rseq_stride(N) + rpad(n), one padding slot every 16 elements, so the
+ * stride-16 reads of the second DFT step and the sequences' starts fall on
+ * different banks */
+__host__ __device__ constexpr int rpad(int n) { return n + (n >> 4); }
+__host__ __device__ constexpr int rseq_stride(int N) { return N + N / 16; }
+/* float2 slots of one buffer: 2 planes x 6 pairs x (GABE_BLOCK / N) rows */
+__host__ __device__ constexpr int rbuf_size(int N) { return 2 * 6 * (GABE_BLOCK / N) * rseq_stride(N); }
+
 template <int N>
 __device__ __forceinline__ void row_r2c(float2 *zl, float2 *zo, const SweepArgs<float> &A, int i, int j0, int npl)
 {
@@ -83,33 +92,34 @@ __device__ __forceinline__ void row_r2c(float2 *zl, float2 *zo, const SweepArgs<
     constexpr int N1 = N == 16 ? 4 : (N <= 64 ? 8 : 16);
     constexpr int N2 = N / N1;
     constexpr int R = GABE_BLOCK / N, SPP = 6 * R, H = N / 2 + 1;   /* sequences per plane */
+    constexpr int SS = rseq_stride(N);
     const int nseq = SPP * npl;
     const float2 *tw = (const float2 *)A.tw;
     for (int task = threadIdx.x; task < nseq * N2; task += GABE_BLOCK) {
         const int sq = task / N2, n2 = task - sq * N2;
-        float2 *b = zl + sq * N;
+        float2 *b = zl + sq * SS;
         float2 x[N1];
 #pragma unroll
-        for (int n1 = 0; n1 < N1; n1++) x[n1] = b[N2 * n1 + n2];
+        for (int n1 = 0; n1 < N1; n1++) x[n1] = b[rpad(N2 * n1 + n2)];
         gabe_fft::dft_reg<N1, float>(x);
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
             float2 v = x[k1];
             if (k1 != 0 && n2 != 0) v = cmul(v, tw[(n2 * k1) % N]);
-            b[N2 * k1 + n2] = v;
+            b[rpad(N2 * k1 + n2)] = v;
         }
     }
     __syncthreads();
     for (int task = threadIdx.x; task < nseq * N1; task += GABE_BLOCK) {
         const int sq = task / N1, k1 = task - sq * N1;
-        const float2 *b = zl + sq * N;
-        float2 *o = zo + sq * N;
+        const float2 *b = zl + sq * SS;
+        float2 *o = zo + sq * SS;
         float2 y[N2];
 #pragma unroll
-        for (int n2 = 0; n2 < N2; n2++) y[n2] = b[N2 * k1 + n2];
+        for (int n2 = 0; n2 < N2; n2++) y[n2] = b[rpad(N2 * k1 + n2)];
         gabe_fft::dft_reg<N2, float>(y);
 #pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) o[k1 + N1 * k2] = y[k2];
+        for (int k2 = 0; k2 < N2; k2++) o[rpad(k1 + N1 * k2)] = y[k2];
     }
     __syncthreads();
     const long comp = (long)N * N * A.kpad;
@@ -117,8 +127,8 @@ __device__ __forceinline__ void row_r2c(float2 *zl, float2 *zo, const SweepArgs<
     for (int task = threadIdx.x; task < nseq * H; task += GABE_BLOCK) {
         const int sq = task / H, m = task - sq * H;
         const int pl = sq / SPP, ps = sq - pl * SPP, pr = ps / R, r = ps - pr * R;
-        const float2 *o = zo + sq * N;
-        const float2 Z = o[m], W = o[m == 0 ? 0 : N - m];
+        const float2 *o = zo + sq * SS;
+        const float2 Z = o[rpad(m)], W = o[rpad(m == 0 ? 0 : N - m)];
         const long row = ((long)(i + pl) * N + (j0 + r)) * A.kpad + m;
         out[(2 * pr) * comp + row] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
         out[(2 * pr + 1) * comp + row] = make_float2(0.5f * (Z.y + W.y), -0.5f * (Z.x - W.x));
@@ -284,10 +294,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                         }
                         const T q3 = (sxx + syy + szz) * (T)(1. / 3.);
                         if constexpr (ROWN > 0) {
-                            float2 *z = zl + (((i - i0) & 1) * 6 + 3 * st) * GABE_BLOCK + threadIdx.x;
+                            /* sequence (plane, pair, row) = ((i - i0) & 1, 3 st + q, t / N) */
+                            constexpr int RR = GABE_BLOCK / ROWN, SS = rseq_stride(ROWN);
+                            const int t = threadIdx.x, r = t / ROWN, kk = t - r * ROWN;
+                            float2 *z = zl + ((((i - i0) & 1) * 6 + 3 * st) * RR + r) * SS + rpad(kk);
                             z[0] = make_float2((sxx - q3) * idx2, sxy * idx2);
-                            z[GABE_BLOCK] = make_float2(sxz * idx2, (syy - q3) * idx2);
-                            z[2 * GABE_BLOCK] = make_float2(syz * idx2, (szz - q3) * idx2);
+                            z[RR * SS] = make_float2(sxz * idx2, (syy - q3) * idx2);
+                            z[2 * RR * SS] = make_float2(syz * idx2, (szz - q3) * idx2);
                         } else {
                             T *S = A.stress + (long)(6 * st) * A.N3 + base + pp;
                             S[0 * A.N3] = (sxx - q3) * idx2;
@@ -322,7 +335,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
 template <int NF, int ROWN>
 __global__ __launch_bounds__(GABE_BLOCK) void k_sweep_rowfft(SweepArgs<float> A)
 {
-    __shared__ float2 zl[2 * 6 * GABE_BLOCK], zo[2 * 6 * GABE_BLOCK];
+    __shared__ float2 zl[rbuf_size(ROWN)], zo[rbuf_size(ROWN)];
     sweep_body<float, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo);
 }
 
@@ -610,14 +623,16 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw(GwArgs<T> G)
 /* reference planes [12][M] (11,12,13,22,23,33 x Re,Im) <-> polarisation pairs
  * [M][2] complex: h_pol = (e_pol : h)/2, h_ij = sum_pol h_pol e_pol_ij */
 template <typename T>
-__global__ __launch_bounds__(GABE_BLOCK) void k_gw_pack(const double *ref, T *pol, int N, long M)
+__global__ __launch_bounds__(GABE_BLOCK) void k_gw_pack(const double *ref, T *pol, ModeLayout lay, long M)
 {
     using C = typename cplx_of<T>::type;
     const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
     if (m >= M) return;
+    const int N = lay.N;
     int px, py, pz;
     long r;
     mode_of(m, N, px, py, pz, r);
+    const long sl = mode_slot(lay, (int)(r / N), (int)(r % N), pz);
     C out[2];
     out[0].x = out[0].y = out[1].x = out[1].y = 0;
     if (px != 0 || py != 0 || pz != 0) {
@@ -634,22 +649,24 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw_pack(const double *ref, T *po
             out[q].y = (T)im;
         }
     }
-    reinterpret_cast<C *>(pol)[2 * m] = out[0];
-    reinterpret_cast<C *>(pol)[2 * m + 1] = out[1];
+    reinterpret_cast<C *>(pol)[2 * sl] = out[0];
+    reinterpret_cast<C *>(pol)[2 * sl + 1] = out[1];
 }
 
 template <typename T>
-__global__ __launch_bounds__(GABE_BLOCK) void k_gw_unpack(const T *pol, double *ref, int N, long M)
+__global__ __launch_bounds__(GABE_BLOCK) void k_gw_unpack(const T *pol, double *ref, ModeLayout lay, long M)
 {
     using C = typename cplx_of<T>::type;
     const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
     if (m >= M) return;
+    const int N = lay.N;
     int px, py, pz;
     long r;
     mode_of(m, N, px, py, pz, r);
+    const long sl = mode_slot(lay, (int)(r / N), (int)(r % N), pz);
     double e[2][6] = {{0., 0., 0., 0., 0., 0.}, {0., 0., 0., 0., 0., 0.}};
     if (px != 0 || py != 0 || pz != 0) pol_basis(px, py, pz, e[0], e[1]);
-    const C a = reinterpret_cast<const C *>(pol)[2 * m], b = reinterpret_cast<const C *>(pol)[2 * m + 1];
+    const C a = reinterpret_cast<const C *>(pol)[2 * sl], b = reinterpret_cast<const C *>(pol)[2 * sl + 1];
     for (int c = 0; c < 6; c++) {
         ref[(long)(2 * c) * M + m] = e[0][c] * (double)a.x + e[1][c] * (double)b.x;
         ref[(long)(2 * c + 1) * M + m] = e[0][c] * (double)a.y + e[1][c] * (double)b.y;
@@ -742,7 +759,7 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_spectra(SpecArgs<T> P)
             if (Lg) {
                 /* sum_ij |l_ij|^2 (off-diagonal components twice, g2spectra.cpp:133-144)
                  * = 2 (|l+|^2 + |lx|^2) for the polarisation pair */
-                const long ms = (P.quirk && kz == 0) ? row + N / 2 : m;
+                const long ms = mode_slot(P.lay, i, j, (P.quirk && kz == 0) ? N / 2 : kz);
                 const C a = Lg[2 * ms], bb = Lg[2 * ms + 1];
                 const double e = 2. * ((double)a.x * (double)a.x + (double)a.y * (double)a.y +
                                        (double)bb.x * (double)bb.x + (double)bb.y * (double)bb.y);
@@ -873,17 +890,17 @@ void launch_gw(const GwArgs<T> &a, hipStream_t s)
 }
 
 template <typename T>
-void launch_gw_pack(const double *ref, T *pol, int N, long M, hipStream_t s)
+void launch_gw_pack(const double *ref, T *pol, ModeLayout lay, long M, hipStream_t s)
 {
     const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
-    hipLaunchKernelGGL((k_gw_pack<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, ref, pol, N, M);
+    hipLaunchKernelGGL((k_gw_pack<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, ref, pol, lay, M);
 }
 
 template <typename T>
-void launch_gw_unpack(const T *pol, double *ref, int N, long M, hipStream_t s)
+void launch_gw_unpack(const T *pol, double *ref, ModeLayout lay, long M, hipStream_t s)
 {
     const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
-    hipLaunchKernelGGL((k_gw_unpack<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, pol, ref, N, M);
+    hipLaunchKernelGGL((k_gw_unpack<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, pol, ref, lay, M);
 }
 
 template <typename T>
@@ -924,8 +941,8 @@ void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
     template void launch_sweep<T>(int, int, bool, const SweepArgs<T> &, dim3, hipStream_t);                   \
     template void launch_sweep_batch<T>(int, int, const SweepArgs<T> *, const int *, int, int, hipStream_t);  \
     template void launch_gw<T>(const GwArgs<T> &, hipStream_t);                                               \
-    template void launch_gw_pack<T>(const double *, T *, int, long, hipStream_t);                            \
-    template void launch_gw_unpack<T>(const T *, double *, int, long, hipStream_t);                          \
+    template void launch_gw_pack<T>(const double *, T *, ModeLayout, long, hipStream_t);                     \
+    template void launch_gw_unpack<T>(const T *, double *, ModeLayout, long, hipStream_t);                   \
     template void launch_tt_out<T>(const T *, int, int, long, long, double, double *, hipStream_t);           \
     template void launch_spectra<T>(const SpecArgs<T> &, hipStream_t);                                        \
     template void launch_effmass<T>(const T *, const T *, int, int, double, double, gabe_model, long,        \

@@ -47,6 +47,21 @@ __device__ __forceinline__ void pol_basis(int px, int py, int pz, double ep[6], 
     pol_components<double>(u, v, ep, ex);
 }
 
+/* Storage order of the tensor-mode arrays h, l (one polarisation pair per
+ * mode).  Lattice order: slot (i N + j) KP + kz, KP = N/2 + 1.  Tiled order,
+ * used with the fused i-transform kernel: [j][kz/4][i][kz%4], so each of its
+ * workgroups (one j, four kz, all i) reads and writes one contiguous block. */
+struct ModeLayout {
+    int N, tiled, nt4;  /* nt4 = ceil(KP / 4) */
+    long KP;
+};
+
+__host__ __device__ __forceinline__ long mode_slot(const ModeLayout &L, int i, int j, int kz)
+{
+    if (!L.tiled) return ((long)i * L.N + j) * L.KP + kz;
+    return (((long)j * L.nt4 + (kz >> 2)) * L.N + i) * 4 + (kz & 3);
+}
+
 __device__ __forceinline__ void mode_of(long m, int N, int &px, int &py, int &pz, long &row)
 {
     const int H = N / 2 + 1;
