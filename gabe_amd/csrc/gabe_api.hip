@@ -193,37 +193,39 @@ int launch_gw_ctx(gabe_ctx *c)
 
 /* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid).
  * with_i = false stops after the j columns (the fused kernel does i). */
-int stress_fft(gabe_ctx *c, bool with_i = true)
+int stress_fft(gabe_ctx *c, bool with_i = true, hipStream_t st = nullptr)
 {
+    if (!st) st = c->stream;
     if (!c->colfft) return run_plan(c, c->fft_stress, c->stress, c->shat);
     if (!c->rowfft) CHK(run_plan(c, c->fft_stress, c->stress, c->shat));   /* rows along k */
     const long N = c->N, KP = N / 2 + 1, kp = c->kpad;
     ColFftView vj{c->shat, kp, N * kp, 0, (int)KP, 1, 12 * N};        /* along j */
     ColFftView vi{c->shat, N * kp, N * N * kp, kp, (int)KP, (int)N, 12}; /* along i */
     if (c->fp32) {
-        launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, c->stream);
-        if (with_i) launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
+        launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, st);
+        if (with_i) launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, st);
     } else {
-        launch_colfft<double>(vj, c->N, c->fft_cw, c->twiddle, c->stream);
-        if (with_i) launch_colfft<double>(vi, c->N, c->fft_cw, c->twiddle, c->stream);
+        launch_colfft<double>(vj, c->N, c->fft_cw, c->twiddle, st);
+        if (with_i) launch_colfft<double>(vi, c->N, c->fft_cw, c->twiddle, st);
     }
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
 
 /* tensor modes of one step: i transform + TT projection + both RK2 stages */
-int gw_modes(gabe_ctx *c)
+int gw_modes(gabe_ctx *c, hipStream_t st = nullptr)
 {
+    if (!st) st = c->stream;
     if (!c->gwfft) return launch_gw_ctx(c);
     const double dx3 = c->dx * c->dx * c->dx;
     if (c->fp32) {
         GwFftArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, (const float *)c->twiddle,
                            c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4};
-        launch_gw_ifft<float>(g, c->stream);
+        launch_gw_ifft<float>(g, st);
     } else {
         GwFftArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss,
                             (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4};
-        launch_gw_ifft<double>(g, c->stream);
+        launch_gw_ifft<double>(g, st);
     }
     HIPCHK(hipGetLastError());
     return GABE_OK;
@@ -240,12 +242,30 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
         if (ev) HIPCHK(hipEventRecord(ev[k], c->stream));
         return GABE_OK;
     };
+    const bool pipe = c->pipe && !ev;
+    /* this slot's spectra and step scalars are free once the side work of
+     * the step two back, which read them, is done */
+    if (pipe) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_gw[c->slot], 0));
     CHK(rec(0));
     CHK(launch_sweep_ctx(c, SW_A, c->gw));
     CHK(rec(1));
     CHK(rec(2));
     CHK(launch_finalize_ctx(c, SW_A, 0));
     CHK(rec(3));
+    if (pipe) {
+        HIPCHK(hipEventRecord(c->ev_fin, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fin, 0));
+        CHK(stress_fft(c, false, c->side));
+        CHK(gw_modes(c, c->side));
+        HIPCHK(hipEventRecord(c->ev_gw[c->slot], c->side));
+        CHK(launch_sweep_ctx(c, SW_B, false));
+        c->cur = 1 - c->cur;
+        c->slot ^= 1;
+        c->shat = c->shat_b[c->slot];
+        c->ss = c->ss_b[c->slot];
+        if (c->closure) CHK(closure_after_step(c));
+        return GABE_OK;
+    }
     const bool overlap = c->gw && !ev && c->side;
     if (overlap) {
         HIPCHK(hipEventRecord(c->ev_fin, c->stream));
@@ -264,6 +284,11 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     else CHK(launch_sweep_ctx(c, SW_B, false));
     CHK(rec(9));
     c->cur = 1 - c->cur;
+    if (c->pipe) {
+        c->slot ^= 1;
+        c->shat = c->shat_b[c->slot];
+        c->ss = c->ss_b[c->slot];
+    }
     if (c->closure) CHK(closure_after_step(c));
     return GABE_OK;
 }
@@ -364,7 +389,17 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->mslots = c->lay.tiled ? (long)c->N * c->lay.nt4 * c->N * 4 : c->M;
     if (const char *e = std::getenv("GABE_FFT_TILE")) c->fft_cw = std::atoi(e) == 32 ? 32 : 16;
     HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
-    HIPCHK(hipMalloc(&c->shat, 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad)));
+    const size_t shat_bytes = 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad);
+    HIPCHK(hipMalloc(&c->shat, shat_bytes));
+    c->shat_b[0] = c->shat;
+    c->pipe = c->rowfft && c->side != nullptr;
+    if (c->pipe) {
+        HIPCHK(hipMalloc(&c->shat_b[1], shat_bytes));
+        for (int q = 0; q < 2; q++) {
+            HIPCHK(hipEventCreateWithFlags(&c->ev_gw[q], hipEventDisableTiming));
+            HIPCHK(hipEventRecord(c->ev_gw[q], c->side));
+        }
+    }
     if (c->alloc_gw) {
         HIPCHK(hipMalloc(&c->h, 2 * c->esz * 2 * c->mslots));
         HIPCHK(hipMalloc(&c->l, 2 * c->esz * 2 * c->mslots));
@@ -389,6 +424,11 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
 
     HIPCHK(hipMalloc(&c->sc, sizeof(DevScalars)));
     HIPCHK(hipMalloc(&c->ss, sizeof(StepScalars)));
+    c->ss_b[0] = c->ss;
+    if (c->pipe) {
+        HIPCHK(hipMalloc(&c->ss_b[1], sizeof(StepScalars)));
+        HIPCHK(hipMemset(c->ss_b[1], 0, sizeof(StepScalars)));
+    }
     DevScalars d{};
     d.a[0] = d.a[1] = 1.;
     HIPCHK(hipMemcpy(c->sc, &d, sizeof d, hipMemcpyHostToDevice));
@@ -431,11 +471,14 @@ void destroy_impl(gabe_ctx *c)
     destroy_plan(c->fft_spec);
     destroy_plan(c->fft_c2r);
     destroy_plan(c->fft_rx);
-    void *bufs[] = {c->fbuf[0], c->fbuf[1], c->vbuf[0], c->vbuf[1], c->stress, c->shat, c->h, c->l,
-                    c->partials, c->sc, c->ss, c->hist, c->spec_out, c->fft_work, c->twiddle, c->rx_buf};
+    void *bufs[] = {c->fbuf[0], c->fbuf[1], c->vbuf[0], c->vbuf[1], c->stress, c->shat_b[0], c->shat_b[1], c->h,
+                    c->l, c->partials, c->sc, c->ss_b[0], c->ss_b[1], c->hist, c->spec_out, c->fft_work,
+                    c->twiddle, c->rx_buf};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    for (hipEvent_t e : c->ev_gw)
+        if (e) (void)hipEventDestroy(e);
     if (c->ev_fin) (void)hipEventDestroy(c->ev_fin);
     if (c->ev_side) (void)hipEventDestroy(c->ev_side);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -828,6 +871,10 @@ int gabe_step(gabe_ctx *c, int nsteps)
     if (!c) return fail(GABE_ERR_ARG, "NULL context");
     if (nsteps < 0) return fail(GABE_ERR_ARG, "nsteps < 0");
     for (int n = 0; n < nsteps; n++) CHK(one_step(c, nullptr));
+    /* the tensor-mode work of the last steps joins the main stream, so every
+     * later synchronisation of it covers the side stream too */
+    if (c->pipe)
+        for (hipEvent_t e : c->ev_gw) HIPCHK(hipStreamWaitEvent(c->stream, e, 0));
     return GABE_OK;
 }
 

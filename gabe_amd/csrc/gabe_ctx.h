@@ -60,6 +60,16 @@ struct gabe_ctx {
     bool own_stream = true;          /* false for ensemble members (the ensemble's stream) */
     hipStream_t side = nullptr;      /* sweep B runs here, beside the stress transform and GW chain */
     hipEvent_t ev_fin = nullptr, ev_side = nullptr;
+    /* pipelined steps (FP32, k rows in sweep A, fused i-transform): the
+     * j-transform and the tensor modes of step n run on the side stream while
+     * sweeps and finalize of step n+1 run on the main one; the stress spectra
+     * and the step scalars alternate between two slots, and ev_gw[s] marks the
+     * side work of the last step that used slot s as done */
+    bool pipe = false;
+    int slot = 0;
+    void *shat_b[2] = {nullptr, nullptr};
+    StepScalars *ss_b[2] = {nullptr, nullptr};
+    hipEvent_t ev_gw[2] = {nullptr, nullptr};
     void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
     void *vbuf[2] = {nullptr, nullptr};
     int cur = 0;
