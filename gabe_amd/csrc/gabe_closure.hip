@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "gabe_ctx.h"
@@ -52,26 +53,30 @@ constexpr int P_B3 = P_W3 + CL_OUT * CL_H * CL_TAPS;
 constexpr int P_TOTAL = P_B3 + CL_OUT;
 static_assert(P_TOTAL == GABE_CLOSURE_NPARAMS, "parameter count of the C ABI");
 
-constexpr int WIDE_STEPS = 14;   /* 27 taps x 16 channels, 2 taps per MFMA */
-constexpr int NARROW_STEPS = 4;  /* 27 taps x 4 channels, 8 taps per MFMA */
-constexpr int CONV_BPW = 4;      /* 16-site blocks per wave */
+/* The K dimension of one MFMA step covers (dz, dx) tap columns of the 3x3
+ * stencil (9 columns) for one halo row; the three dy taps of a column are
+ * three separate weight operands, so one activation read serves three
+ * output rows. */
+constexpr int TAP_COLS = 9;
+constexpr int WIDE_STEPS = 5;    /* 9 columns x 16 channels, 2 columns per MFMA (the last half empty) */
+constexpr int NARROW_STEPS = 2;  /* 9 columns x 4 channels, 8 columns per MFMA (the second step one column) */
 constexpr int WG_ROWS = 2;       /* lattice rows per weight-gradient slab */
 constexpr int WG_SLABS = 8;      /* slabs per weight-gradient workgroup */
 
 enum ConvEpi { EPI_HIDDEN_TRAIN, EPI_HIDDEN, EPI_BWD, EPI_OUT };
 
-/* packed MFMA B operands, [steps][64 lanes] x 8 BF16: the three forward
- * layers, then the backward-data convolutions of layers 3 and 2; biases are
- * kept as [3][16] FP32, zero padded */
-constexpr size_t PK_F1 = 0, PK_F2 = PK_F1 + NARROW_STEPS * 64, PK_F3 = PK_F2 + WIDE_STEPS * 64,
-                 PK_B3 = PK_F3 + WIDE_STEPS * 64, PK_B2 = PK_B3 + NARROW_STEPS * 64,
-                 PK_TOTAL = PK_B2 + WIDE_STEPS * 64;
+/* packed MFMA weight operands, [steps][3 dy][64 lanes] x 8 BF16: the three
+ * forward layers, then the backward-data convolutions of layers 3 and 2;
+ * biases are kept as [3][16] FP32, zero padded */
+constexpr size_t PK_F1 = 0, PK_F2 = PK_F1 + NARROW_STEPS * 3 * 64, PK_F3 = PK_F2 + WIDE_STEPS * 3 * 64,
+                 PK_B3 = PK_F3 + WIDE_STEPS * 3 * 64, PK_B2 = PK_B3 + NARROW_STEPS * 3 * 64,
+                 PK_TOTAL = PK_B2 + WIDE_STEPS * 3 * 64;
 
 struct ConvArgs {
     const bf16 *in;
     const bf16x8 *w;
     const float *bias;
-    int N;
+    int N, depth;            /* depth: planes per workgroup column */
     long N3;
     bf16 *z_out, *a_out;     /* HIDDEN: [N3][16] */
     const bf16 *zin;         /* BWD: pre-activation of the layer below, [N3][16] */
@@ -89,10 +94,13 @@ __device__ __forceinline__ long site_of(int i, int j, int k, int t, int N, long 
     return (long)ii * N2 + (long)jj * N + kk;
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+/* logistic function with the hardware reciprocal (1 ulp): the results are
+ * rounded to BF16 right after */
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = 1.f / (1.f + __expf(-z));
+    const float s = sigm(z);
     return s * (1.f + z * (1.f - s));
 }
 
@@ -105,62 +113,202 @@ __device__ __forceinline__ int logical_wg()
     return nwg % GABE_NXCD == 0 ? (hw % GABE_NXCD) * (nwg / GABE_NXCD) + hw / GABE_NXCD : hw;
 }
 
+/* A workgroup owns a column of TX (k) x TY (j) sites and `depth` planes
+ * along i, and marches down it TZ planes (two per wave) at a time.  The input
+ * planes with their 1-site halo live channel-last in a ring of RING plane
+ * slots in LDS: the TZ + 2 planes the current tile reads and the TZ the next
+ * tile adds, which are fetched into registers while the current tile's MFMAs
+ * run.
+ * Each wave keeps its weight operands in registers and walks the TY + 2 halo
+ * rows of its plane: the activations of one (halo row, tap-column step) are
+ * one aligned LDS read and feed the MFMAs of every output row y = yy - dy.
+ * The MFMA computes D[channel][site], so a lane ends with 4 consecutive
+ * channels of one site. */
+constexpr int TX = 16, TY = 4, ZW = 2, TZ = 4 * ZW;   /* ZW planes per wave */
+constexpr int HX = TX + 2, HY = TY + 2, PLANE = HX * HY;
+constexpr int RING = 2 * TZ + 2;
+
+__device__ __forceinline__ int wrap(int x, int N) { return x < 0 ? x + N : (x >= N ? x - N : x); }
+
 template <int CI, int EPI>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv(ConvArgs A)
 {
     constexpr int NSTEP = CI == 16 ? WIDE_STEPS : NARROW_STEPS;
-    __shared__ bf16x8 wl[NSTEP * 64];
-    __shared__ float bl[16];
+    constexpr int CH = CI == 16 ? 2 : 1;          /* copy chunks per site: 16-byte halves, or one 8-byte site */
+    constexpr int CE = CI / CH;                   /* bf16 per chunk */
+    using Chunk = typename std::conditional<CI == 16, bf16x8, bf16x4>::type;
+    constexpr int NEW = TZ * PLANE * CH;          /* chunks of the planes one tile adds */
+    constexpr int NLD = (NEW + 255) / 256;
+    __shared__ __align__(16) bf16 tin[RING * PLANE * CI];
     if (A.gate && A.gate[0] == 0) return;
-    for (int q = threadIdx.x; q < NSTEP * 64; q += 256) wl[q] = A.w[q];
-    if (threadIdx.x < 16) bl[threadIdx.x] = A.bias ? A.bias[threadIdx.x] : 0.f;
-    __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const int N = A.N;
+    const int N = A.N, D = A.depth;
     const long N2 = (long)N * N;
     const int wg = logical_wg();
-    for (int bb = 0; bb < CONV_BPW; bb++) {
-        const long s0 = (((long)wg * 4 + wave) * CONV_BPW + bb) * 16;
-        const int i = (int)(s0 / N2), j = (int)((s0 / N) % N), k0 = (int)(s0 % N);
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int ntx = N / TX, nty = N / TY;
+    const int k0 = (wg % ntx) * TX, j0 = ((wg / ntx) % nty) * TY, ib = (wg / (ntx * nty)) * D;
+
+    /* chunk c of the planes starting at plane offset o0 (plane ib - 1 + o) */
+    auto src = [&](int c, int o0) -> long {
+        const int pl = c / (PLANE * CH), rem = c - pl * (PLANE * CH);
+        const int hs = rem / CH, part = rem - hs * CH;
+        const int xx = hs % HX, yy = hs / HX;
+        const int ii = wrap(ib - 1 + o0 + pl, N), jj = wrap(j0 + yy - 1, N), kk = wrap(k0 + xx - 1, N);
+        return ((long)ii * N2 + (long)jj * N + kk) * CI + part * CE;
+    };
+    auto dst = [&](int c, int o0) -> int {
+        const int pl = c / (PLANE * CH), rem = c - pl * (PLANE * CH);
+        return ((o0 + pl) % RING) * PLANE * CI + rem * CE;
+    };
+    for (int c = tid; c < (TZ + 2) * PLANE * CH; c += 256) *(Chunk *)(tin + dst(c, 0)) = *(const Chunk *)(A.in + src(c, 0));
+
+    /* this thread's chunks of the TZ planes each step adds, fixed through the
+     * march: plane within the step and LDS offset within a plane slot
+     * (packed), and the (j, k, half) part of the global offset */
+    int cpl[NLD], cjk[NLD];
 #pragma unroll
-        for (int s = 0; s < NSTEP; s++) {
-            bf16x8 a;
-            if constexpr (CI == 16) {
-                const int t = 2 * s + (g >> 1);
-                if (t < CL_TAPS) {
-                    a = *(const bf16x8 *)(A.in + site_of(i, j, k0 + r, t, N, N2) * 16 + 8 * (g & 1));
-                } else {
-                    for (int e = 0; e < 8; e++) a[e] = (bf16)0.f;
-                }
-            } else {
-                const int t0 = 8 * s + 2 * g;
-                bf16x4 lo, hi;
-                for (int e = 0; e < 4; e++) lo[e] = hi[e] = (bf16)0.f;
-                if (t0 < CL_TAPS) lo = *(const bf16x4 *)(A.in + site_of(i, j, k0 + r, t0, N, N2) * 4);
-                if (t0 + 1 < CL_TAPS) hi = *(const bf16x4 *)(A.in + site_of(i, j, k0 + r, t0 + 1, N, N2) * 4);
-                a = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wl[s * 64 + lane], acc, 0, 0, 0);
+    for (int m = 0; m < NLD; m++) {
+        const int c = tid + 256 * m;
+        const int pl = c / (PLANE * CH), rem = c - pl * (PLANE * CH);
+        const int hs = rem / CH, part = rem - hs * CH;
+        const int jj = wrap(j0 + hs / HX - 1, N), kk = wrap(k0 + hs % HX - 1, N);
+        cpl[m] = c < NEW ? (pl << 16) | (rem * CE) : -1;
+        cjk[m] = (jj * N + kk) * CI + part * CE;
+    }
+    const int N2CI = N * N * CI;
+
+    bf16x8 wr[NSTEP][3];
+#pragma unroll
+    for (int s = 0; s < NSTEP; s++)
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) wr[s][dy] = A.w[(s * 3 + dy) * 64 + lane];
+    float bias[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) bias[q] = A.bias ? A.bias[4 * g + q] : 0.f;
+
+    /* this lane's K slots: per step one tap column (8 of its 16 channels) or
+     * two (4 channels each): the column's dz, and the offset of its dx and
+     * channels within a plane slot.  Slots past the 9 columns carry zero
+     * weights; they read the last column, so no operand needs a select. */
+    constexpr int NC = CI == 16 ? 1 : 2;
+    int cdz[NSTEP][NC], coff[NSTEP][NC];
+#pragma unroll
+    for (int s = 0; s < NSTEP; s++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            int col = CI == 16 ? 2 * s + (g >> 1) : 8 * s + 2 * g + c;
+            col = col < TAP_COLS ? col : TAP_COLS - 1;
+            cdz[s][c] = col / 3;
+            coff[s][c] = (r + col % 3) * CI + (CI == 16 ? 8 * (g & 1) : 0);
         }
-        /* lane holds D[site 4g+q][channel r] */
+    /* the weights and biases are loop invariant: wait for them here, so the
+     * only vector loads in flight inside the march are the next planes */
+    __builtin_amdgcn_s_waitcnt(0x0F70);   /* vmcnt(0) */
+    __syncthreads();
+
+    const int nit = D / TZ;
+    for (int n = 0; n < nit; n++) {
+        const bool more = n + 1 < nit;
+        Chunk pre[NLD];
+        const int o0 = TZ * n + TZ + 2;   /* plane offset of the first plane the next step adds */
+        if (more) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const long site = s0 + 4 * g + q;
-            const float v = acc[q] + bl[r];
-            if constexpr (EPI == EPI_HIDDEN_TRAIN) {
-                A.z_out[site * 16 + r] = (bf16)v;
-                A.a_out[site * 16 + r] = (bf16)silu(v);
-            } else if constexpr (EPI == EPI_HIDDEN) {
-                A.a_out[site * 16 + r] = (bf16)silu(v);
-            } else if constexpr (EPI == EPI_BWD) {
-                const float z = (float)A.zin[site * 16 + r];
-                A.g_out[site * 16 + r] = (bf16)(acc[q] * dsilu(z));
-            } else if constexpr (EPI == EPI_OUT) {
-                if (r < CL_OUT) A.y_out[site * 2 + r] = v;
+            for (int m = 0; m < NLD; m++)
+                if (cpl[m] >= 0) {
+                    const int ii = wrap(ib - 1 + o0 + (cpl[m] >> 16), N);
+                    pre[m] = *(const Chunk *)(A.in + (__umul24(ii, N2CI) + cjk[m]));
+                }
+        }
+#pragma unroll
+        for (int zp = 0; zp < ZW; zp++) {
+            const int z = ZW * wave + zp;
+            /* plane slots of this wave's planes i = ib + TZ n + z + dz - 1 */
+            int slot[3];
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) slot[dz] = ((TZ * n + z + dz) % RING) * PLANE * CI;
+            int ad[NSTEP][NC];
+#pragma unroll
+            for (int s = 0; s < NSTEP; s++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const int d = cdz[s][c];
+                    ad[s][c] = (d == 0 ? slot[0] : (d == 1 ? slot[1] : slot[2])) + coff[s][c];
+                }
+            /* the activation operands of halo row yy + 1 are read from LDS while
+             * the MFMAs of row yy run */
+            auto read_row = [&](bf16x8 *av, int yy) {
+#pragma unroll
+                for (int s = 0; s < NSTEP; s++) {
+                    if constexpr (CI == 16) {
+                        av[s] = *(const bf16x8 *)(tin + ad[s][0] + yy * HX * 16);
+                    } else {
+                        const bf16x4 lo = *(const bf16x4 *)(tin + ad[s][0] + yy * HX * 4);
+                        const bf16x4 hi = *(const bf16x4 *)(tin + ad[s][1] + yy * HX * 4);
+                        av[s] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                    }
+                }
+            };
+            bf16x8 avc[NSTEP], avn[NSTEP];
+            read_row(avc, 0);
+            f32x4 acc[TY];
+#pragma unroll
+            for (int y = 0; y < TY; y++) acc[y] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int yy = 0; yy < TY + 2; yy++) {
+                if (yy + 1 < TY + 2) read_row(avn, yy + 1);
+#pragma unroll
+                for (int s = 0; s < NSTEP; s++)
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++) {
+                        const int y = yy - dy;
+                        if (y >= 0 && y < TY)
+                            acc[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[s][dy], avc[s], acc[y], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int s = 0; s < NSTEP; s++) avc[s] = avn[s];
             }
+            /* lane holds D[channels 4g .. 4g + 3][site k0 + r of row (i, j0 + y)] */
+            const int i = ib + TZ * n + z;
+#pragma unroll
+            for (int y = 0; y < TY; y++) {
+                const int site = __umul24(__umul24(i, N) + j0 + y, N) + k0 + r;
+                float v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = acc[y][q] + bias[q];
+                if constexpr (EPI == EPI_HIDDEN_TRAIN || EPI == EPI_HIDDEN) {
+                    bf16x4 h;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) h[q] = (bf16)silu(v[q]);
+                    *(bf16x4 *)(A.a_out + site * 16 + 4 * g) = h;
+                    if constexpr (EPI == EPI_HIDDEN_TRAIN) {
+                        bf16x4 zq;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) zq[q] = (bf16)v[q];
+                        *(bf16x4 *)(A.z_out + site * 16 + 4 * g) = zq;
+                    }
+                } else if constexpr (EPI == EPI_BWD) {
+                    const bf16x4 zq = *(const bf16x4 *)(A.zin + site * 16 + 4 * g);
+                    bf16x4 o;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) o[q] = (bf16)(acc[y][q] * dsilu((float)zq[q]));
+                    *(bf16x4 *)(A.g_out + site * 16 + 4 * g) = o;
+                } else if constexpr (EPI == EPI_OUT) {
+                    if (g == 0) *(float2 *)(A.y_out + site * 2) = make_float2(v[0], v[1]);
+                }
+            }
+        }
+        if (more) {
+            const int sb = o0 % RING;
+#pragma unroll
+            for (int m = 0; m < NLD; m++)
+                if (cpl[m] >= 0) {
+                    int sl = sb + (cpl[m] >> 16);
+                    sl = sl >= RING ? sl - RING : sl;
+                    *(Chunk *)(tin + sl * PLANE * CI + (cpl[m] & 0xFFFF)) = pre[m];
+                }
+            __syncthreads();
         }
     }
 }
@@ -371,21 +519,22 @@ __global__ void k_wgrad_reduce(const float *part, const float *bias_part, int nw
 }
 
 /* ----------------------------------------------------------- packing */
-/* B operand value of a conv with input channels cin (padded to 16 or 4 slots)
- * and output channels cout, from W[co][ci][t] (or, flipped, from the layer
- * whose weights are Wf[ci][co][26 - t]). */
-__device__ float packed_weight(const float *W, int wco, int wci, bool flip, int wide, int s, int lane, int e)
+/* Weight operand value, step sd = 3 s + dy, of a conv with input channels
+ * cin (padded to 16 or 4 slots) and output channels cout, from W[co][ci][t]
+ * (or, flipped, from the layer whose weights are Wf[ci][co][26 - t]). */
+__device__ float packed_weight(const float *W, int wco, int wci, bool flip, int wide, int sd, int lane, int e)
 {
-    const int n = lane & 15, g = lane >> 4;
-    int t, ci;
+    const int n = lane & 15, g = lane >> 4, s = sd / 3, dy = sd % 3;
+    int col, ci;
     if (wide) {
-        t = 2 * s + (g >> 1);
+        col = 2 * s + (g >> 1);
         ci = 8 * (g & 1) + e;
     } else {
-        t = 8 * s + 2 * g + (e >> 2);
+        col = 8 * s + 2 * g + (e >> 2);
         ci = e & 3;
     }
-    if (t >= CL_TAPS) return 0.f;
+    if (col >= TAP_COLS) return 0.f;
+    const int t = 9 * (col / 3) + 3 * dy + col % 3;
     if (!flip) {   /* conv from wci channels to wco channels */
         if (n >= wco || ci >= wci) return 0.f;
         return W[(n * wci + ci) * CL_TAPS + t];
@@ -592,7 +741,7 @@ struct gabe_closure {
     long flap_n3 = 0;
     std::vector<bf16 *> xs;     /* dataset: inputs [N3][4] */
     std::vector<float *> ts;    /* dataset: targets [2][N3] */
-    int conv_wg = 0, wgrad_wg = 0;
+    int wgrad_wg = 0;
 };
 
 struct ClosureRun {
@@ -616,13 +765,26 @@ int cl_check_n(int N)
     return GABE_OK;
 }
 
-/* one workgroup per 4 waves x CONV_BPW blocks of 16 sites */
-int conv_workgroups(long N3) { return (int)(N3 / (16 * 4 * CONV_BPW)); }
+/* planes per workgroup column: the longest march (a multiple of TZ dividing
+ * N) that still leaves every CU as many workgroups as fit on it at once (2
+ * of the 16-channel kernels, 4 of the 4-channel ones) */
+int conv_depth(int N, int CI)
+{
+    const long target = CI == 16 ? 512 : 1024;
+    int best = TZ;
+    for (int d = TZ; d <= N; d += TZ)
+        if (N % d == 0 && (long)N * N * N / ((long)TX * TY * d) >= target) best = d;
+    return best;
+}
+
+int conv_workgroups(int N, int CI) { return (int)((long)N * N * N / ((long)TX * TY * conv_depth(N, CI))); }
 
 template <int CI, int EPI>
 int launch_conv(const ConvArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_conv<CI, EPI>), dim3(conv_workgroups(a.N3)), dim3(256), 0, s, a);
+    ConvArgs b = a;
+    b.depth = conv_depth(a.N, CI);
+    hipLaunchKernelGGL((k_conv<CI, EPI>), dim3(conv_workgroups(a.N, CI)), dim3(256), 0, s, b);
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
@@ -747,7 +909,6 @@ int create_closure(gabe_closure *cl, int N, double dt, unsigned seed)
     cl->N = N;
     cl->N3 = (long)N * N * N;
     cl->dt = dt;
-    cl->conv_wg = conv_workgroups(cl->N3);
     cl->wgrad_wg = (int)((long)N * (N / WG_ROWS) / WG_SLABS);
     HIPCHK(hipStreamCreate(&cl->stream));
     const long N3 = cl->N3;
