@@ -83,16 +83,9 @@ struct ConvArgs {
     bf16 *g_out;             /* BWD: [N3][16] */
     float *y_out;            /* OUT: [N3][2] */
     const int *gate;         /* skip everything when gate[0] == 0 */
+    const float *fin[4];     /* 4-channel input as FP32 arrays (used instead of in when set) */
+    float fscale[4];
 };
-
-__device__ __forceinline__ long site_of(int i, int j, int k, int t, int N, long N2)
-{
-    int ii = i + t / 9 - 1, jj = j + (t / 3) % 3 - 1, kk = k + t % 3 - 1;
-    ii = ii < 0 ? ii + N : (ii >= N ? ii - N : ii);
-    jj = jj < 0 ? jj + N : (jj >= N ? jj - N : jj);
-    kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
-    return (long)ii * N2 + (long)jj * N + kk;
-}
 
 /* logistic function with the hardware reciprocal (1 ulp): the results are
  * rounded to BF16 right after */
@@ -162,7 +155,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int pl = c / (PLANE * CH), rem = c - pl * (PLANE * CH);
         return ((o0 + pl) % RING) * PLANE * CI + rem * CE;
     };
-    for (int c = tid; c < (TZ + 2) * PLANE * CH; c += 256) *(Chunk *)(tin + dst(c, 0)) = *(const Chunk *)(A.in + src(c, 0));
+    /* one chunk at element offset off of the input: BF16 activations, or for
+     * the first layer of a step correction the four FP32 lattice arrays,
+     * scaled and rounded to BF16 here */
+    auto fetch = [&](int off) -> Chunk {
+        if constexpr (CI == 4) {
+            if (A.fin[0]) {
+                const int site = off >> 2;
+                bf16x4 x;
+#pragma unroll
+                for (int q = 0; q < 4; q++) x[q] = (bf16)(A.fin[q][site] * A.fscale[q]);
+                return x;
+            }
+        }
+        return *(const Chunk *)(A.in + off);
+    };
+    for (int c = tid; c < (TZ + 2) * PLANE * CH; c += 256) *(Chunk *)(tin + dst(c, 0)) = fetch((int)src(c, 0));
 
     /* this thread's chunks of the TZ planes each step adds, fixed through the
      * march: plane within the step and LDS offset within a plane slot
@@ -218,7 +226,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int m = 0; m < NLD; m++)
                 if (cpl[m] >= 0) {
                     const int ii = wrap(ib - 1 + o0 + (cpl[m] >> 16), N);
-                    pre[m] = *(const Chunk *)(A.in + (__umul24(ii, N2CI) + cjk[m]));
+                    pre[m] = fetch(__umul24(ii, N2CI) + cjk[m]);
                 }
         }
 #pragma unroll
@@ -315,16 +323,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 /* ------------------------------------------------- the Laplacian output */
 /* lap of both channels of u [N3][2] at site s */
+/* site index -> (i, j, k) in 32-bit arithmetic (lattices here hold < 2^31 sites) */
+__device__ __forceinline__ void site_ijk(long s, int N, int &i, int &j, int &k)
+{
+    const unsigned n = (unsigned)N, q = (unsigned)s / n;
+    k = (int)((unsigned)s - q * n);
+    i = (int)(q / n);
+    j = (int)(q - (unsigned)i * n);
+}
+
 __device__ __forceinline__ float2 lap2(const float *u, long s, int N, long N2)
 {
-    const int i = (int)(s / N2), j = (int)((s / N) % N), k = (int)(s % N);
-    const long ip = (i + 1 == N ? 0 : i + 1) * N2 + (long)j * N + k, im = (i == 0 ? N - 1 : i - 1) * N2 + (long)j * N + k;
-    const long jp = (long)i * N2 + (long)(j + 1 == N ? 0 : j + 1) * N + k, jm = (long)i * N2 + (long)(j == 0 ? N - 1 : j - 1) * N + k;
-    const long kp = (long)i * N2 + (long)j * N + (k + 1 == N ? 0 : k + 1), km = (long)i * N2 + (long)j * N + (k == 0 ? N - 1 : k - 1);
+    int i, j, k;
+    site_ijk(s, N, i, j, k);
+    const int n2 = (int)N2, ss = (int)s;
+    const int ip = i + 1 == N ? ss - i * n2 : ss + n2, im = i == 0 ? ss + (N - 1) * n2 : ss - n2;
+    const int jp = j + 1 == N ? ss - j * N : ss + N, jm = j == 0 ? ss + (N - 1) * N : ss - N;
+    const int kp = k + 1 == N ? ss - k : ss + 1, km = k == 0 ? ss + N - 1 : ss - 1;
     const float2 c = *(const float2 *)(u + 2 * s);
     float2 sum = *(const float2 *)(u + 2 * ip);
-    for (long q : {im, jp, jm, kp, km}) {
-        const float2 v = *(const float2 *)(u + 2 * q);
+    for (int q : {im, jp, jm, kp, km}) {
+        const float2 v = *(const float2 *)(u + 2 * (long)q);
         sum.x += v.x;
         sum.y += v.y;
     }
@@ -602,7 +621,8 @@ __global__ void k_make_x(const float *f0, const float *f1, const float *v0, cons
  * out_c = (sum of the 6 neighbours - 6 f_c) / dx^2 */
 __device__ __forceinline__ float lap_soa(const float *f, long s, int N, long N2)
 {
-    const int i = (int)(s / N2), j = (int)((s / N) % N), k = (int)(s % N);
+    int i, j, k;
+    site_ijk(s, N, i, j, k);
     const long base = s - k;
     float sum = f[(i + 1 == N ? s - (long)i * N2 : s + N2)] + f[(i == 0 ? s + (long)(N - 1) * N2 : s - N2)];
     sum += f[(j + 1 == N ? s - (long)j * N : s + N)] + f[(j == 0 ? s + (long)(N - 1) * N : s - N)];
@@ -748,7 +768,7 @@ struct ClosureRun {
     int N = 0;                     /* weights and scales are copies: the closure may go */
     bf16x8 *pk = nullptr;
     float *bias = nullptr;
-    bf16 *x = nullptr, *a1 = nullptr, *a2 = nullptr;
+    bf16 *a1 = nullptr, *a2 = nullptr;
     float *u = nullptr, *w = nullptr;   /* network output u and lap u, [N3][2] */
     int *gate = nullptr;           /* [0] active */
     long long *fallback = nullptr; /* step count when the gate closed */
@@ -974,10 +994,12 @@ int closure_after_step(gabe_ctx *c)
                        R->fallback, R->rmax);
     HIPCHK(hipGetLastError());
     const float *f = (const float *)c->fbuf[c->cur], *v = (const float *)c->vbuf[c->cur];
-    hipLaunchKernelGGL(k_make_x, dim3(nblocks(c->N3, 256)), dim3(256), 0, c->stream, f, f + c->N3, v, v + c->N3,
-                       R->inv_s[0], R->inv_s[1], R->inv_s[2], R->inv_s[3], c->N3, R->x, R->gate);
-    HIPCHK(hipGetLastError());
-    ConvArgs a = conv_base(R->N, R->x, R->pk + PK_F1, R->bias);
+    ConvArgs a = conv_base(R->N, nullptr, R->pk + PK_F1, R->bias);
+    a.fin[0] = f;
+    a.fin[1] = f + c->N3;
+    a.fin[2] = v;
+    a.fin[3] = v + c->N3;
+    for (int q = 0; q < 4; q++) a.fscale[q] = R->inv_s[q];
     a.gate = R->gate;
     a.a_out = R->a1;
     CHK((launch_conv<4, EPI_HIDDEN>(a, c->stream)));
@@ -1002,7 +1024,7 @@ void closure_release(gabe_ctx *c)
 {
     ClosureRun *R = c->closure;
     if (!R) return;
-    void *bufs[] = {R->pk, R->bias, R->x, R->a1, R->a2, R->u, R->w, R->gate, R->fallback, R->rmax};
+    void *bufs[] = {R->pk, R->bias, R->a1, R->a2, R->u, R->w, R->gate, R->fallback, R->rmax};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete R;
@@ -1304,7 +1326,7 @@ int gabe_closure_attach(gabe_ctx *c, gabe_closure *cl, double residual_threshold
     const long N3 = c->N3;
     int rc = GABE_OK;
     if (hipMalloc(&R->pk, sizeof(bf16x8) * PK_TOTAL) != hipSuccess || hipMalloc(&R->bias, sizeof(float) * 48) ||
-        hipMalloc(&R->x, sizeof(bf16) * 4 * N3) || hipMalloc(&R->a1, sizeof(bf16) * 16 * N3) ||
+        hipMalloc(&R->a1, sizeof(bf16) * 16 * N3) ||
         hipMalloc(&R->a2, sizeof(bf16) * 16 * N3) || hipMalloc(&R->u, sizeof(float) * 2 * N3) ||
         hipMalloc(&R->w, sizeof(float) * 2 * N3) ||
         hipMalloc(&R->gate, sizeof(int)) ||

@@ -144,6 +144,58 @@ def test_zero_closure_is_the_plain_step_and_fallback_switches_it_off():
     assert not np.array_equal(A.download_state()[1], plain.download_state()[1])   # it was applied
 
 
+def stored_arrays(lat):
+    """The FP32 lattice's field and velocity arrays as stored on the device
+    (fluctuations about the FP64 means): the tail of its checkpoint blob when
+    the lattice carries no tensor modes."""
+    n = 2 * lat.N ** 3
+    tail = np.frombuffer(lat.checkpoint()[-8 * n:].tobytes(), dtype=np.float32)
+    shp = (2, lat.N, lat.N, lat.N)
+    return tail[:n].reshape(shp), tail[n:].reshape(shp)
+
+
+def test_step_correction_is_the_network_kick():
+    """After one step with the closure attached the fields equal the plain
+    step's, and the stored velocities differ by dt s_out[c] y_c / a^2: y the
+    network output (the forward call, itself pinned to the CPU network above)
+    for the post-step stored arrays (fluctuations about the FP64 means, as in
+    the training samples) scaled by 1 / s_in and rounded to BF16; the step
+    path reads the FP32 arrays inside its first convolution."""
+    setup = small_setup()
+    cfg = setup.coarse_cfg
+    assert cfg.calc_gws == 0
+    N = cfg.N
+    A = host.initialized_lattice(cfg, precision=GABE_FP32)
+    B = host.initialized_lattice(cfg, precision=GABE_FP32)
+    f0, v0 = stored_arrays(B)
+    s_in = [float(np.sqrt((f0[0].astype(np.float64) ** 2).mean())) or 1.0,
+            float(np.sqrt((f0[1].astype(np.float64) ** 2).mean())) or 1.0,
+            float(np.sqrt((v0[0].astype(np.float64) ** 2).mean())) or 1.0,
+            float(np.sqrt((v0[1].astype(np.float64) ** 2).mean())) or 1.0]
+    s_out = [2e-2, 3e-2]
+    cl = cm.Closure(N, cfg.dt, seed=9)
+    cl.set_scales(s_in, s_out)
+    cm.attach(A, cl, 1e300)
+    A.step(1)
+    B.step(1)
+    for a, b in zip(A.download_state()[0], B.download_state()[0]):
+        assert np.array_equal(a, b)
+    fB, vB = stored_arrays(B)
+    _, vA = stored_arrays(A)
+    inv = [np.float32(1.0 / s) for s in s_in]
+    x = np.stack([fB[0] * inv[0], fB[1] * inv[1], vB[0] * inv[2], vB[1] * inv[3]], axis=-1)
+    y = cl.forward(x).reshape(N, N, N, 2).astype(np.float64)
+    a = A.get_scalars()["a"][0]
+    ia2 = float(np.float32(1.0 / (a * a)))
+    eps = float(np.finfo(np.float32).eps)
+    for c in range(2):
+        kick = float(np.float32(cfg.dt * s_out[c])) * ia2 * y[..., c]
+        got = vA[c].astype(np.float64) - vB[c]
+        assert np.abs(kick).max() > 1e3 * eps * np.abs(vB[c]).max()    # the kick is well resolved
+        assert np.all(np.abs(got - kick) <= 2 * eps * (np.abs(vB[c]) + np.abs(kick)) + 1e-30)
+    cl.close()
+
+
 def test_training_reduces_loss_and_the_closure_beats_the_plain_step():
     setup = small_setup()
     cl = cm.Closure(setup.coarse_cfg.N, setup.coarse_cfg.dt, seed=7)
