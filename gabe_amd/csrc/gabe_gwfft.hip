@@ -31,14 +31,25 @@ constexpr int load_batch(int n)
     return 1;
 }
 
+/* LDS row of lattice index i within a tile: one spare row after every N2
+ * rows.  Rows are CW complex wide; the first DFT step's lanes read N2
+ * consecutive rows, the second step's lanes rows N2 apart, and with this
+ * spacing both (and the loads, stores and projections) fall on distinct
+ * banks. */
+template <int N2>
+__device__ __forceinline__ int trow(int i) { return i + i / N2; }
+/* trow(N2 a + b) for b < N2 */
+template <int N2>
+__device__ __forceinline__ int trow2(int a, int b) { return (N2 + 1) * a + b; }
+
 /* One round of an RK2 stage: components 6 ST + C0 .. 6 ST + C0 + CPR - 1 of
  * the source (CPR = 6: the whole stage; CPR = 3: half of it, the projection
  * being linear in the components): load the CPR N x CW tiles, transform along
- * i (two steps, twiddles read from the first tile's padding column), return
- * to natural order, and project the owned modes on e+ and ex into
- * acc[mode][pol]. */
+ * i (two steps, twiddles from twl), return to natural order, and project the
+ * owned modes on e+ and ex into acc[mode][pol]. */
 template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0>
-__device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CPR][N1 * N2][CW + 1],
+__device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CPR][N1 * N2 + N1][CW],
+                                             const typename cplx_t<T>::type *twl,
                                              const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
                                              const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
                                              const T (&bv)[(N1 * N2 * CW + NT - 1) / NT][3],
@@ -70,7 +81,7 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
             for (int t = 0; t < LB; t++) {
                 const int e = threadIdx.x + (t0 + t) * NT;
                 const int r = e / (N * CW), rem = e % (N * CW);
-                if (e < CPR * N * CW) tile[r][rem / CW][rem % CW] = v[t];
+                if (e < CPR * N * CW) tile[r][trow<N2>(rem / CW)][rem % CW] = v[t];
             }
         }
         __syncthreads();
@@ -82,13 +93,13 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
                 const int r = task / (N2 * CW), rem = task % (N2 * CW), col = rem % CW, n2 = rem / CW;
                 C x[N1];
 #pragma unroll
-                for (int n1 = 0; n1 < N1; n1++) x[n1] = tile[r][N2 * n1 + n2][col];
+                for (int n1 = 0; n1 < N1; n1++) x[n1] = tile[r][trow2<N2>(n1, n2)][col];
                 dft_reg<N1, T>(x);
 #pragma unroll
                 for (int k1 = 0; k1 < N1; k1++) {
                     C w = x[k1];
-                    if (k1 != 0 && n2 != 0) w = cmul(w, tile[0][(n2 * k1) % N][CW]);
-                    tile[r][N2 * k1 + n2][col] = w;
+                    if (k1 != 0 && n2 != 0) w = cmul(w, twl[(n2 * k1) % N]);
+                    tile[r][trow2<N2>(k1, n2)][col] = w;
                 }
             }
         }
@@ -101,7 +112,7 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
             if (task < CPR * N1 * CW) {
                 const int r = task / (N1 * CW), rem = task % (N1 * CW), col = rem % CW, k1 = rem / CW;
 #pragma unroll
-                for (int n2 = 0; n2 < N2; n2++) y[t][n2] = tile[r][N2 * k1 + n2][col];
+                for (int n2 = 0; n2 < N2; n2++) y[t][n2] = tile[r][trow2<N2>(k1, n2)][col];
                 dft_reg<N2, T>(y[t]);
             }
         }
@@ -112,7 +123,7 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
             if (task < CPR * N1 * CW) {
                 const int r = task / (N1 * CW), rem = task % (N1 * CW), col = rem % CW, k1 = rem / CW;
 #pragma unroll
-                for (int k2 = 0; k2 < N2; k2++) tile[r][k1 + N1 * k2][col] = y[t][k2];
+                for (int k2 = 0; k2 < N2; k2++) tile[r][trow<N2>(k1 + N1 * k2)][col] = y[t][k2];
             }
         }
         __syncthreads();
@@ -135,7 +146,7 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
                     const T w = ia == ib ? (T)0.5 : (T)1;
                     const T ep = w * (ua * ub - va * vb);
                     const T ex = w * (ua * vb + va * ub);
-                    const C S = tile[r][i][col];
+                    const C S = tile[r][trow<N2>(i)][col];
                     acc[u][0].x += ep * S.x;
                     acc[u][0].y += ep * S.y;
                     acc[u][1].x += ex * S.x;
@@ -147,15 +158,24 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
     }
 }
 
+/* waves per SIMD the register allocation must leave room for: the 256^3
+ * default tile fits 3 without spilling */
+constexpr int gw_min_waves(int N1, int N2, int CW, int CPR, int NT)
+{
+    return N1 == 16 && N2 == 16 && CW == 4 && CPR == 3 && NT == 256 ? 3 : 1;
+}
+
 template <typename T, int N1, int N2, int CW, int CPR, int NT>
-__global__ __launch_bounds__(NT) void k_gw_ifft(GwFftArgs<T> A)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(gw_min_waves(N1, N2, CW, CPR, NT))))
+void k_gw_ifft(GwFftArgs<T> A)
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2, KP = N / 2 + 1;
     constexpr int NCH = (KP + CW - 1) / CW;
     constexpr int MPT = (N * CW + NT - 1) / NT;        /* modes owned per thread */
-    static_assert(sizeof(C) * CPR * N * (CW + 1) <= 160 * 1024, "tile exceeds the LDS of one CU");
-    __shared__ C tile[CPR][N][CW + 1];
+    static_assert(sizeof(C) * (CPR * (N + N1) * CW + N) <= 160 * 1024, "tile exceeds the LDS of one CU");
+    __shared__ C tile[CPR][N + N1][CW];
+    __shared__ C twl[N];
 
     /* XCD-aware order: adjacent kz chunks share 128-byte lines, so keep them
      * on one XCD (one L2), working at the same time */
@@ -167,8 +187,8 @@ __global__ __launch_bounds__(NT) void k_gw_ifft(GwFftArgs<T> A)
     const long SM = (long)N * N * A.kpad, rstride = (long)N * A.kpad;
     const C *Sh = reinterpret_cast<const C *>(A.shat) + ((long)j * A.kpad + (long)q * CW);
 
-    /* twiddles exp(-2 pi i e/N) in the padding column of the first tile */
-    for (int e = threadIdx.x; e < N; e += NT) tile[0][e][CW] = reinterpret_cast<const C *>(A.tw)[e];
+    /* twiddles exp(-2 pi i e/N) */
+    for (int e = threadIdx.x; e < N; e += NT) twl[e] = reinterpret_cast<const C *>(A.tw)[e];
 
     /* owned modes: e = tid + u NT -> (i = e / CW, col = e % CW) */
     T bu[MPT][3], bv[MPT][3];
@@ -185,10 +205,10 @@ __global__ __launch_bounds__(NT) void k_gw_ifft(GwFftArgs<T> A)
 #pragma unroll
         for (int p = 0; p < 2; p++) { acc0[u][p].x = acc0[u][p].y = (T)0; acc1[u][p].x = acc1[u][p].y = (T)0; }
     }
-    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0>(tile, Sh, SM, rstride, width, bu, bv, acc0);
-    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3>(tile, Sh, SM, rstride, width, bu, bv, acc0);
-    stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(tile, Sh, SM, rstride, width, bu, bv, acc1);
-    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3>(tile, Sh, SM, rstride, width, bu, bv, acc1);
+    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0>(tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
+    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3>(tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
+    stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3>(tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
 
     /* both midpoint-RK2 stages of the owned modes (derivs, g2evolutionpert.cpp:486-501) */
     const StepScalars *ss = A.ss;
