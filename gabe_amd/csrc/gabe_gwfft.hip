@@ -158,15 +158,17 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
     }
 }
 
-/* waves per SIMD the register allocation must leave room for: the 256^3
- * default tile fits 3 without spilling */
-constexpr int gw_min_waves(int N1, int N2, int CW, int CPR, int NT)
+/* waves per SIMD the register allocation must leave room for: the FP32
+ * 256^3 tile of 4 columns x 3 components on 256 threads fits 3 without
+ * spilling */
+constexpr int gw_min_waves(int tsize, int N1, int N2, int CW, int CPR, int NT)
 {
-    return N1 == 16 && N2 == 16 && CW == 4 && CPR == 3 && NT == 256 ? 3 : 1;
+    return tsize == 4 && N1 == 16 && N2 == 16 && CW == 4 && CPR == 3 && NT == 256 ? 3 : 1;
 }
 
 template <typename T, int N1, int N2, int CW, int CPR, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(gw_min_waves(N1, N2, CW, CPR, NT))))
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(gw_min_waves((int)sizeof(T), N1, N2, CW, CPR, NT))))
 void k_gw_ifft(GwFftArgs<T> A)
 {
     using C = typename cplx_t<T>::type;
@@ -256,7 +258,7 @@ void launch_cfg(const GwFftArgs<T> &a, hipStream_t s)
 
 } // namespace
 
-bool gabe_gwfft_supported(int N, bool fp32) { return fp32 && N >= 16 && N <= 256 && (N & (N - 1)) == 0; }
+bool gabe_gwfft_supported(int N, bool) { return N >= 16 && N <= 256 && (N & (N - 1)) == 0; }
 
 template <>
 void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
@@ -286,5 +288,23 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
     }
 }
 
+/* FP64: complex values are 16 bytes, so the tiles are half as wide in kz to
+ * keep the LDS and register footprint of the FP32 ones */
 template <>
-void launch_gw_ifft<double>(const GwFftArgs<double> &, hipStream_t) {}
+void launch_gw_ifft<double>(const GwFftArgs<double> &a, hipStream_t s)
+{
+    const char *v = std::getenv("GABE_GWFFT_TILE");
+    const int t = v ? std::atoi(v) : 0;
+    switch (a.N) {
+    case 16: launch_cfg<double, 4, 4, 8, 6, 256>(a, s); break;
+    case 32: launch_cfg<double, 8, 4, 8, 6, 256>(a, s); break;
+    case 64: launch_cfg<double, 8, 8, 4, 6, 256>(a, s); break;
+    case 128: launch_cfg<double, 16, 8, 4, 6, 512>(a, s); break;
+    case 256:
+        if (t == 1) launch_cfg<double, 16, 16, 2, 3, 256>(a, s);
+        else if (t == 2) launch_cfg<double, 16, 16, 4, 3, 256>(a, s);
+        else launch_cfg<double, 16, 16, 2, 6, 256>(a, s);
+        break;
+    default: break;
+    }
+}

@@ -258,21 +258,26 @@ def test_gw_upload_keeps_the_tt_part():
     assert np.abs(h2).max() < 1e-13 * np.abs(junk).max()
 
 
-@pytest.mark.parametrize("N", [16, 64])
-def test_fused_gw_transform_matches_separate_passes(N, monkeypatch):
-    """FP32: the i transform fused with the mode update gives the same tensor
-    modes and fields as the separate i pass followed by the mode kernel."""
+@pytest.mark.parametrize("precision,N", [(GABE_FP32, 16), (GABE_FP32, 32), (GABE_FP32, 64), (GABE_FP32, 128),
+                                         (GABE_FP32, 256), (GABE_FP64, 16), (GABE_FP64, 32), (GABE_FP64, 64),
+                                         (GABE_FP64, 128), (GABE_FP64, 256)])
+def test_fused_gw_transform_matches_separate_passes(precision, N, monkeypatch):
+    """The i transform fused with the mode update gives the same tensor modes
+    and fields as the separate i pass followed by the mode kernel, in both
+    storage forms (FP64: the k rows come from the library transform, FP32:
+    from sweep A)."""
     cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
     o = seeded_oracle(cfg, amp=2e-3)
-    fused = lattice_from_oracle(o, cfg, GABE_FP32)
+    fused = lattice_from_oracle(o, cfg, precision)
     monkeypatch.setenv("GABE_UNFUSED_GW", "1")
-    sep = lattice_from_oracle(o, cfg, GABE_FP32)
+    sep = lattice_from_oracle(o, cfg, precision)
     monkeypatch.delenv("GABE_UNFUSED_GW")
     fused.step(12)
     sep.step(12)
     hf, lf = fused.download_gw()
     hs, ls = sep.download_gw()
-    assert rel_err(lf, ls) < 1e-5 and rel_err(hf, hs) < 1e-5
+    tol = 1e-5 if precision == GABE_FP32 else 1e-12
+    assert rel_err(lf, ls) < tol and rel_err(hf, hs) < tol
     ff, _ = fused.download_state()
     fs, _ = sep.download_state()
     assert np.array_equal(ff, fs)   # the field path does not depend on the GW kernels
