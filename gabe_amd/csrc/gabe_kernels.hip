@@ -82,29 +82,35 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
  * different banks */
 __host__ __device__ constexpr int rpad(int n) { return n + (n >> 4); }
 __host__ __device__ constexpr int rseq_stride(int N) { return N + N / 16; }
-/* float2 slots of one buffer: 2 planes x 6 pairs x (GABE_BLOCK / N) rows */
-__host__ __device__ constexpr int rbuf_size(int N) { return 2 * 6 * (GABE_BLOCK / N) * rseq_stride(N); }
+/* planes per transform call: 2 in FP32, 1 in FP64 (same LDS bytes) */
+template <typename T>
+__host__ __device__ constexpr int row_planes() { return sizeof(T) == 4 ? 2 : 1; }
+/* complex slots of one buffer: row_planes x 6 pairs x (GABE_BLOCK / N) rows */
+template <typename T>
+__host__ __device__ constexpr int rbuf_size(int N) { return row_planes<T>() * 6 * (GABE_BLOCK / N) * rseq_stride(N); }
 
-template <int N>
-__device__ __forceinline__ void row_r2c(float2 *zl, float2 *zo, const SweepArgs<float> &A, int i, int j0, int npl)
+template <typename T, int N>
+__device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, typename gabe_fft::cplx_t<T>::type *zo,
+                                       const SweepArgs<T> &A, int i, int j0, int npl)
 {
     using gabe_fft::cmul;
+    using C = typename gabe_fft::cplx_t<T>::type;
     constexpr int N1 = N == 16 ? 4 : (N <= 64 ? 8 : 16);
     constexpr int N2 = N / N1;
     constexpr int R = GABE_BLOCK / N, SPP = 6 * R, H = N / 2 + 1;   /* sequences per plane */
     constexpr int SS = rseq_stride(N);
     const int nseq = SPP * npl;
-    const float2 *tw = (const float2 *)A.tw;
+    const C *tw = (const C *)A.tw;
     for (int task = threadIdx.x; task < nseq * N2; task += GABE_BLOCK) {
         const int sq = task / N2, n2 = task - sq * N2;
-        float2 *b = zl + sq * SS;
-        float2 x[N1];
+        C *b = zl + sq * SS;
+        C x[N1];
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) x[n1] = b[rpad(N2 * n1 + n2)];
-        gabe_fft::dft_reg<N1, float>(x);
+        gabe_fft::dft_reg<N1, T>(x);
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
-            float2 v = x[k1];
+            C v = x[k1];
             if (k1 != 0 && n2 != 0) v = cmul(v, tw[(n2 * k1) % N]);
             b[rpad(N2 * k1 + n2)] = v;
         }
@@ -112,32 +118,39 @@ __device__ __forceinline__ void row_r2c(float2 *zl, float2 *zo, const SweepArgs<
     __syncthreads();
     for (int task = threadIdx.x; task < nseq * N1; task += GABE_BLOCK) {
         const int sq = task / N1, k1 = task - sq * N1;
-        const float2 *b = zl + sq * SS;
-        float2 *o = zo + sq * SS;
-        float2 y[N2];
+        const C *b = zl + sq * SS;
+        C *o = zo + sq * SS;
+        C y[N2];
 #pragma unroll
         for (int n2 = 0; n2 < N2; n2++) y[n2] = b[rpad(N2 * k1 + n2)];
-        gabe_fft::dft_reg<N2, float>(y);
+        gabe_fft::dft_reg<N2, T>(y);
 #pragma unroll
         for (int k2 = 0; k2 < N2; k2++) o[rpad(k1 + N1 * k2)] = y[k2];
     }
     __syncthreads();
     const long comp = (long)N * N * A.kpad;
-    float2 *out = (float2 *)A.shat;
+    C *out = (C *)A.shat;
+    const T h = (T)0.5;
     for (int task = threadIdx.x; task < nseq * H; task += GABE_BLOCK) {
         const int sq = task / H, m = task - sq * H;
         const int pl = sq / SPP, ps = sq - pl * SPP, pr = ps / R, r = ps - pr * R;
-        const float2 *o = zo + sq * SS;
-        const float2 Z = o[rpad(m)], W = o[rpad(m == 0 ? 0 : N - m)];
+        const C *o = zo + sq * SS;
+        const C Z = o[rpad(m)], W = o[rpad(m == 0 ? 0 : N - m)];
         const long row = ((long)(i + pl) * N + (j0 + r)) * A.kpad + m;
-        out[(2 * pr) * comp + row] = make_float2(0.5f * (Z.x + W.x), 0.5f * (Z.y - W.y));
-        out[(2 * pr + 1) * comp + row] = make_float2(0.5f * (Z.y + W.y), -0.5f * (Z.x - W.x));
+        C s0, s1;
+        s0.x = h * (Z.x + W.x);
+        s0.y = h * (Z.y - W.y);
+        s1.x = h * (Z.y + W.y);
+        s1.y = -h * (Z.x - W.x);
+        out[(2 * pr) * comp + row] = s0;
+        out[(2 * pr + 1) * comp + row] = s1;
     }
 }
 
 template <typename T, int NF, int MODE, bool GW, int ROWN = 0>
-__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy, float2 *zl = nullptr,
-                                           float2 *zo = nullptr)
+__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy,
+                                           typename gabe_fft::cplx_t<T>::type *zl = nullptr,
+                                           typename gabe_fft::cplx_t<T>::type *zo = nullptr)
 {
     constexpr int NS = gabe_nsums(NF);
     const int N = A.N;
@@ -294,13 +307,18 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                         }
                         const T q3 = (sxx + syy + szz) * (T)(1. / 3.);
                         if constexpr (ROWN > 0) {
-                            /* sequence (plane, pair, row) = ((i - i0) & 1, 3 st + q, t / N) */
-                            constexpr int RR = GABE_BLOCK / ROWN, SS = rseq_stride(ROWN);
+                            /* sequence (plane, pair, row) = ((i - i0) mod row_planes, 3 st + q, t / N) */
+                            using C = typename gabe_fft::cplx_t<T>::type;
+                            constexpr int RR = GABE_BLOCK / ROWN, SS = rseq_stride(ROWN), PPC = row_planes<T>();
                             const int t = threadIdx.x, r = t / ROWN, kk = t - r * ROWN;
-                            float2 *z = zl + ((((i - i0) & 1) * 6 + 3 * st) * RR + r) * SS + rpad(kk);
-                            z[0] = make_float2((sxx - q3) * idx2, sxy * idx2);
-                            z[RR * SS] = make_float2(sxz * idx2, (syy - q3) * idx2);
-                            z[2 * RR * SS] = make_float2(syz * idx2, (szz - q3) * idx2);
+                            C *z = zl + ((((i - i0) % PPC) * 6 + 3 * st) * RR + r) * SS + rpad(kk);
+                            C z0, z1, z2;
+                            z0.x = (sxx - q3) * idx2; z0.y = sxy * idx2;
+                            z1.x = sxz * idx2;        z1.y = (syy - q3) * idx2;
+                            z2.x = syz * idx2;        z2.y = (szz - q3) * idx2;
+                            z[0] = z0;
+                            z[RR * SS] = z1;
+                            z[2 * RR * SS] = z2;
                         } else {
                             T *S = A.stress + (long)(6 * st) * A.N3 + base + pp;
                             S[0 * A.N3] = (sxx - q3) * idx2;
@@ -315,10 +333,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             }
         }
         if constexpr (ROWN > 0) {
-            if (((i - i0) & 1) || i + 1 == i1) {
-                const int npl = ((i - i0) & 1) ? 2 : 1;
+            constexpr int PPC = row_planes<T>();
+            if ((i - i0) % PPC == PPC - 1 || i + 1 == i1) {
+                const int npl = (i - i0) % PPC + 1;
                 __syncthreads();
-                row_r2c<ROWN>(zl, zo, A, i + 1 - npl, bx * (GABE_BLOCK / ROWN), npl);
+                row_r2c<T, ROWN>(zl, zo, A, i + 1 - npl, bx * (GABE_BLOCK / ROWN), npl);
                 __syncthreads();
             }
         }
@@ -331,12 +350,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
     if (MODE != SW_B) block_store_partials<NS>(acc, A.partials, A.nblk, bid);
 }
 
-/* sweep A with the stress rows transformed in the workgroup (FP32, N 16..256) */
-template <int NF, int ROWN>
-__global__ __launch_bounds__(GABE_BLOCK) void k_sweep_rowfft(SweepArgs<float> A)
+/* sweep A with the stress rows transformed in the workgroup (N 16..256) */
+template <typename T, int NF, int ROWN>
+__global__ __launch_bounds__(GABE_BLOCK) void k_sweep_rowfft(SweepArgs<T> A)
 {
-    __shared__ float2 zl[rbuf_size(ROWN)], zo[rbuf_size(ROWN)];
-    sweep_body<float, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo);
+    using C = typename gabe_fft::cplx_t<T>::type;
+    __shared__ C zl[rbuf_size<T>(ROWN)], zo[rbuf_size<T>(ROWN)];
+    sweep_body<T, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo);
 }
 
 template <typename T, int NF, int MODE, bool GW>
@@ -819,15 +839,15 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_add_real(T *dst, const double *a
 } // namespace
 
 /* ---------------------------------------------------------------- launchers */
-template <int NF>
-static bool launch_rowfft(const SweepArgs<float> &a, dim3 grid, hipStream_t s)
+template <typename T, int NF>
+static bool launch_rowfft(const SweepArgs<T> &a, dim3 grid, hipStream_t s)
 {
     switch (a.rowfft) {
-    case 16: hipLaunchKernelGGL((k_sweep_rowfft<NF, 16>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
-    case 32: hipLaunchKernelGGL((k_sweep_rowfft<NF, 32>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
-    case 64: hipLaunchKernelGGL((k_sweep_rowfft<NF, 64>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
-    case 128: hipLaunchKernelGGL((k_sweep_rowfft<NF, 128>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
-    case 256: hipLaunchKernelGGL((k_sweep_rowfft<NF, 256>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 16: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 16>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 32: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 32>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 64: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 64>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 128: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 128>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 256: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 256>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
     default: return false;
     }
 }
@@ -835,9 +855,7 @@ static bool launch_rowfft(const SweepArgs<float> &a, dim3 grid, hipStream_t s)
 template <typename T, int NF>
 static void launch_sweep_nf(int mode, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s)
 {
-    if constexpr (std::is_same<T, float>::value) {
-        if (mode == SW_A && gw && a.rowfft && launch_rowfft<NF>(a, grid, s)) return;
-    }
+    if (mode == SW_A && gw && a.rowfft && launch_rowfft<T, NF>(a, grid, s)) return;
     if (mode == SW_ENERGY) hipLaunchKernelGGL((k_sweep<T, NF, SW_ENERGY, false>), grid, dim3(GABE_BLOCK), 0, s, a);
     else if (mode == SW_A && gw) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, true>), grid, dim3(GABE_BLOCK), 0, s, a);
     else if (mode == SW_A) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, false>), grid, dim3(GABE_BLOCK), 0, s, a);

@@ -283,23 +283,26 @@ def test_fused_gw_transform_matches_separate_passes(precision, N, monkeypatch):
     assert np.array_equal(ff, fs)   # the field path does not depend on the GW kernels
 
 
-@pytest.mark.parametrize("N", [16, 32, 64, 128, 256])
-def test_rows_transformed_in_sweep_match_library_rows(N, monkeypatch):
-    """FP32: sweep A transforming its own k-rows of the stress (one complex
+@pytest.mark.parametrize("precision,N", [(GABE_FP32, 16), (GABE_FP32, 32), (GABE_FP32, 64), (GABE_FP32, 128),
+                                         (GABE_FP32, 256), (GABE_FP64, 16), (GABE_FP64, 64), (GABE_FP64, 256)])
+def test_rows_transformed_in_sweep_match_library_rows(precision, N, monkeypatch):
+    """Sweep A transforming its own k-rows of the stress (one complex
     transform per pair of real rows, split into both half-spectra) gives the
-    tensor modes of the library row transform; the stress transform it feeds
-    is checked directly through gabe_tt_source."""
+    tensor modes of the library row transform, in both storage forms; the
+    stress transform it feeds is checked directly through gabe_tt_source."""
     cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
     o = seeded_oracle(cfg, amp=2e-3)
-    fused = lattice_from_oracle(o, cfg, GABE_FP32)
+    fused = lattice_from_oracle(o, cfg, precision)
     monkeypatch.setenv("GABE_LIBRARY_ROWS", "1")
-    lib = lattice_from_oracle(o, cfg, GABE_FP32)
+    lib = lattice_from_oracle(o, cfg, precision)
     monkeypatch.delenv("GABE_LIBRARY_ROWS")
+    fp32 = precision == GABE_FP32
     for s in (0, 1):
         tf, tl = fused.tt_source(s), lib.tt_source(s)
-        assert rel_err(tf, tl) < 2e-6, s
+        assert rel_err(tf, tl) < (2e-6 if fp32 else 1e-13), s
     fused.step(6)
     lib.step(6)
     hf, lf = fused.download_gw()
     hl, ll = lib.download_gw()
-    assert rel_err(lf, ll) < 1e-5 and rel_err(hf, hl) < 1e-5
+    tol = 1e-5 if fp32 else 1e-12
+    assert rel_err(lf, ll) < tol and rel_err(hf, hl) < tol
