@@ -438,6 +438,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     HIPCHK(hipMemset(c->hist, 0, sizeof(double) * GABE_HIST_COLS * c->hist_cap));
     const int nb = gabe_numbins(c->N);
     HIPCHK(hipMalloc(&c->spec_out, sizeof(double) * nb * (GABE_MAXF + 2)));
+    HIPCHK(hipMalloc(&c->spec_part, sizeof(double) * nb * gabe_spec_chunks(c->N) * (GABE_MAXF + 2)));
 
     if (c->alloc_gw) {
         if (c->colfft) {
@@ -472,7 +473,7 @@ void destroy_impl(gabe_ctx *c)
     destroy_plan(c->fft_c2r);
     destroy_plan(c->fft_rx);
     void *bufs[] = {c->fbuf[0], c->fbuf[1], c->vbuf[0], c->vbuf[1], c->stress, c->shat_b[0], c->shat_b[1], c->h,
-                    c->l, c->partials, c->sc, c->ss_b[0], c->ss_b[1], c->hist, c->spec_out, c->fft_work,
+                    c->l, c->partials, c->sc, c->ss_b[0], c->ss_b[1], c->hist, c->spec_out, c->spec_part, c->fft_work,
                     c->twiddle, c->rx_buf};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -902,6 +903,8 @@ int gabe_spectra(gabe_ctx *c, double *out)
         a.l = (const float *)c->l;
         for (int f = 0; f < nf; f++) a.dc[f] = d.pbar[f] * (double)c->N3;
         a.nf = nf; a.N = N; a.numbins = nb; a.quirk = c->p.gw_spectrum_quirk; a.M = c->M; a.out = c->spec_out;
+        a.part = c->spec_part;
+        a.nch = gabe_spec_chunks(N);
         a.lay = c->lay;
         launch_spectra<float>(a, c->stream);
     } else {
@@ -909,6 +912,8 @@ int gabe_spectra(gabe_ctx *c, double *out)
         a.fhat = (const double *)c->shat;
         a.l = (const double *)c->l;
         a.nf = nf; a.N = N; a.numbins = nb; a.quirk = c->p.gw_spectrum_quirk; a.M = c->M; a.out = c->spec_out;
+        a.part = c->spec_part;
+        a.nch = gabe_spec_chunks(N);
         a.lay = c->lay;
         launch_spectra<double>(a, c->stream);
     }

@@ -86,6 +86,7 @@ struct gabe_ctx {
     double *hist = nullptr;
     int hist_cap = 0;
     double *spec_out = nullptr;
+    double *spec_part = nullptr;   /* per-chunk partial sums of the spectra */
     FftPlan fft_stress, fft_spec, fft_c2r;
     bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
     bool gwfft = false;          /* i columns fused with the tensor-mode update */

@@ -98,8 +98,14 @@ struct SpecArgs {
     double dc[GABE_MAXF];   /* added to the real part of each field's k=0 mode */
     int nf, N, numbins, quirk;
     long M;
+    double *part;       /* [numbins][nch][nf + 2] per-chunk partial sums */
+    int nch;            /* px chunks per shell, gabe_spec_chunks(N) */
     double *out;        /* [numbins][nf + 2]: per-field sum w|F|^2, sum w, sum w E_gw */
 };
+
+/* px values per spectrum chunk, and chunks per shell: the widest shell spans N values of px */
+constexpr int GABE_SPEC_PX = 4;
+inline int gabe_spec_chunks(int N) { return (N + GABE_SPEC_PX - 1) / GABE_SPEC_PX; }
 
 template <typename T>
 void launch_sweep(int mode, int nf, bool gw, const SweepArgs<T> &a, dim3 grid, hipStream_t s);

@@ -736,10 +736,12 @@ __device__ __forceinline__ int isqrt_floor(long x)
     return (int)r;
 }
 
-/* One workgroup per bin b: the modes with floor(|n|) = b (g2spectra.cpp:68-92,
- * 126-184) are enumerated column by column ((px,py) fixed, kz contiguous), so
- * every bin is summed by one workgroup in a fixed order: the result does not
- * depend on scheduling.  Weights: 2 for 1 <= kz <= N/2-1, 1 for kz in {0, N/2}. */
+/* Workgroup (b, c) takes bin b's modes with floor(|n|) = b (g2spectra.cpp:68-92,
+ * 126-184) in px chunk c, enumerated column by column ((px,py) fixed, kz
+ * contiguous); k_spectra_reduce then adds the chunks of each bin in chunk
+ * order, so the result does not depend on scheduling.  Splitting the shells
+ * spreads the large ones over many workgroups.  Weights: 2 for
+ * 1 <= kz <= N/2-1, 1 for kz in {0, N/2}. */
 template <typename T>
 __global__ __launch_bounds__(GABE_BLOCK) void k_spectra(SpecArgs<T> P)
 {
@@ -752,10 +754,12 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_spectra(SpecArgs<T> P)
     const int R = b + 1;
     const int lo = max(-R, -N / 2 + 1), hi = min(R, N / 2);
     const int W = hi - lo + 1;
+    /* this workgroup's chunk of the shell: GABE_SPEC_PX values of px */
+    const int px0 = lo + GABE_SPEC_PX * (int)blockIdx.y, npx = max(0, min(GABE_SPEC_PX, hi + 1 - px0));
     const long b2 = (long)b * b, r2max = (long)(b + 1) * (b + 1) - 1;
     double acc[GABE_MAXF + 2] = {0., 0., 0., 0.};
-    for (long q = threadIdx.x; q < (long)W * W; q += GABE_BLOCK) {
-        const int px = lo + (int)(q / W), py = lo + (int)(q % W);
+    for (long q = threadIdx.x; q < (long)npx * W; q += GABE_BLOCK) {
+        const int px = px0 + (int)(q / W), py = lo + (int)(q % W);
         const long r2 = (long)px * px + (long)py * py;
         if (r2 > r2max) continue;
         int kzlo = 0;
@@ -796,8 +800,20 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_spectra(SpecArgs<T> P)
     __syncthreads();
     if ((int)threadIdx.x < nf + 2) {
         const int q = threadIdx.x;
-        P.out[(long)b * (nf + 2) + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+        P.part[((long)b * P.nch + blockIdx.y) * (nf + 2) + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
     }
+}
+
+/* per-shell sums over the chunks, in chunk order (bit-reproducible) */
+__global__ __launch_bounds__(GABE_BLOCK) void k_spectra_reduce(const double *part, int numbins, int nch, int nq,
+                                                                double *out)
+{
+    const int t = blockIdx.x * GABE_BLOCK + threadIdx.x;
+    if (t >= numbins * nq) return;
+    const int b = t / nq, q = t - b * nq;
+    double s = 0.;
+    for (int c = 0; c < nch; c++) s += part[((long)b * nch + c) * nq + q];
+    out[(long)b * nq + q] = s;
 }
 
 /* --------------------------------------------------------------- helpers */
@@ -931,7 +947,10 @@ void launch_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx
 template <typename T>
 void launch_spectra(const SpecArgs<T> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_spectra<T>), dim3(a.numbins), dim3(GABE_BLOCK), 0, s, a);
+    hipLaunchKernelGGL((k_spectra<T>), dim3(a.numbins, a.nch), dim3(GABE_BLOCK), 0, s, a);
+    const int nq = a.nf + 2;
+    hipLaunchKernelGGL(k_spectra_reduce, dim3((a.numbins * nq + GABE_BLOCK - 1) / GABE_BLOCK), dim3(GABE_BLOCK), 0, s,
+                       a.part, a.numbins, a.nch, nq, a.out);
 }
 
 template <typename T>
