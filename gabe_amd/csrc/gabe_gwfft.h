@@ -21,8 +21,8 @@ struct GwFftArgs {
     int nt4;            /* kz tiles of 4 of the tiled mode order */
 };
 
-/* whether the fused kernel exists for this size and storage type: FP32 or
- * FP64, N a power of two in 16..256 */
+/* whether the fused kernel exists for this size and storage type: N a power
+ * of two in 16..512 (FP32) or 16..256 (FP64) */
 bool gabe_gwfft_supported(int N, bool fp32);
 
 template <typename T>

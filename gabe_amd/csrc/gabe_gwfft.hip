@@ -258,7 +258,7 @@ void launch_cfg(const GwFftArgs<T> &a, hipStream_t s)
 
 } // namespace
 
-bool gabe_gwfft_supported(int N, bool) { return N >= 16 && N <= 256 && (N & (N - 1)) == 0; }
+bool gabe_gwfft_supported(int N, bool fp32) { return N >= 16 && N <= (fp32 ? 512 : 256) && (N & (N - 1)) == 0; }
 
 template <>
 void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
@@ -282,6 +282,16 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
         else if (t == 5) launch_cfg<float, 16, 16, 4, 3, 512>(a, s);
         else if (t == 6) launch_cfg<float, 16, 16, 8, 6, 1024>(a, s);
         else launch_cfg<float, 16, 16, 4, 6, 512>(a, s);
+        break;
+    }
+    case 512: {
+        const char *v = std::getenv("GABE_GWFFT_TILE");
+        const int t = v ? std::atoi(v) : 0;
+        /* 2 columns x 6 components on 256 threads measured fastest of these */
+        if (t == 1) launch_cfg<float, 32, 16, 4, 3, 512>(a, s);
+        else if (t == 2) launch_cfg<float, 32, 16, 2, 6, 512>(a, s);
+        else if (t == 3) launch_cfg<float, 32, 16, 2, 3, 256>(a, s);
+        else launch_cfg<float, 32, 16, 2, 6, 256>(a, s);
         break;
     }
     default: break;

@@ -10,6 +10,8 @@ Tolerances:
 """
 import math
 
+import os
+
 import numpy as np
 import pytest
 
@@ -281,6 +283,30 @@ def test_fused_gw_transform_matches_separate_passes(precision, N, monkeypatch):
     ff, _ = fused.download_state()
     fs, _ = sep.download_state()
     assert np.array_equal(ff, fs)   # the field path does not depend on the GW kernels
+
+
+def test_fused_gw_transform_matches_separate_passes_512():
+    """FP32 at 512^3 (library k rows, fused i transform + modes): the same
+    tensor modes as the separate i pass and mode kernel, on the reference's
+    own initial conditions generated on the device."""
+    from gabe_amd import host
+    cfg = configs.lambda_phi4_config(N=512, calc_gws=1)
+    fused = host.initialized_lattice(cfg, precision=GABE_FP32)
+    os.environ["GABE_UNFUSED_GW"] = "1"
+    try:
+        sep = host.initialized_lattice(cfg, precision=GABE_FP32)
+    finally:
+        del os.environ["GABE_UNFUSED_GW"]
+    fused.step(3)
+    sep.step(3)
+    hf, lf = fused.download_gw()
+    hs, ls = sep.download_gw()
+    assert rel_err(lf, ls) < 1e-6 and rel_err(hf, hs) < 1e-6
+    ff, _ = fused.download_state()
+    fs, _ = sep.download_state()
+    assert np.array_equal(ff, fs)
+    fused.close()
+    sep.close()
 
 
 @pytest.mark.parametrize("precision,N", [(GABE_FP32, 16), (GABE_FP32, 32), (GABE_FP32, 64), (GABE_FP32, 128),
