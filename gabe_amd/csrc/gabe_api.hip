@@ -381,7 +381,10 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->colfft = c->alloc_gw && gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
     c->kpad = c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1;
     c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
-    c->rowfft = c->gwfft && c->N >= 16 && c->N <= GABE_BLOCK && !std::getenv("GABE_LIBRARY_ROWS");
+    /* k rows in sweep A: up to GABE_BLOCK sites per row in both forms; 512-site
+     * rows (one 512-thread workgroup each) in FP32 */
+    c->rowfft = c->gwfft && c->N >= 16 && (c->N <= GABE_BLOCK || (c->N == 512 && c->fp32)) &&
+                !std::getenv("GABE_LIBRARY_ROWS");
     c->lay.N = c->N;
     c->lay.KP = c->N / 2 + 1;
     c->lay.nt4 = (int)((c->lay.KP + 3) / 4);

@@ -47,11 +47,15 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-/* fixed-order block reduction of NS sums; one partial per workgroup per sum */
-template <int NS>
+/* fixed-order block reduction of NS sums; one partial per workgroup per sum.
+ * A workgroup of WIDE x GABE_BLOCK threads stands for WIDE consecutive
+ * GABE_BLOCK workgroups: it writes its sum into the first of their slots and
+ * zeros into the others, so the finalize reads the same nblk slots. */
+template <int NS, int WIDE = 1>
 __device__ void block_store_partials(double (&acc)[NS], double *partials, int nblk, int bid)
 {
-    __shared__ double red[GABE_BLOCK / 64][NS];
+    constexpr int NWV = WIDE * GABE_BLOCK / 64;
+    __shared__ double red[NWV][NS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -61,12 +65,15 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
     __syncthreads();
     if (threadIdx.x < NS) {
         const int s = threadIdx.x;
-        partials[(long)s * nblk + bid] = (red[0][s] + red[1][s]) + (red[2][s] + red[3][s]);
+        double tot = (red[0][s] + red[1][s]) + (red[2][s] + red[3][s]);
+        if constexpr (WIDE == 2) tot += (red[4][s] + red[5][s]) + (red[6][s] + red[7][s]);
+        partials[(long)s * nblk + WIDE * bid] = tot;
+        for (int q = 1; q < WIDE; q++) partials[(long)s * nblk + WIDE * bid + q] = 0.;
     }
 }
 
 /* ------------------------------------------------------------------ sweeps */
-/* Row R2C of the 12 stress components of the GABE_BLOCK / N k-rows one sweep
+/* Row R2C of the 12 stress components of the row_block(N) / N k-rows one sweep
  * workgroup holds, for npl (1 or 2) consecutive planes from plane i, written
  * into the padded half-spectrum rows of shat (the library row transform it
  * replaces).  Each stage's components arrive in LDS as three complex rows,
@@ -82,12 +89,16 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
  * different banks */
 __host__ __device__ constexpr int rpad(int n) { return n + (n >> 4); }
 __host__ __device__ constexpr int rseq_stride(int N) { return N + N / 16; }
-/* planes per transform call: 2 in FP32, 1 in FP64 (same LDS bytes) */
+/* threads of a row-transforming sweep workgroup: it holds at least one whole
+ * k-row (512-site rows take a 512-thread workgroup) */
+__host__ __device__ constexpr int row_block(int N) { return N > GABE_BLOCK ? N : GABE_BLOCK; }
+/* planes per transform call: 2 in FP32, 1 in FP64 or for 512-site rows (the
+ * same LDS bytes) */
 template <typename T>
-__host__ __device__ constexpr int row_planes() { return sizeof(T) == 4 ? 2 : 1; }
-/* complex slots of one buffer: row_planes x 6 pairs x (GABE_BLOCK / N) rows */
+__host__ __device__ constexpr int row_planes(int N) { return sizeof(T) == 4 && N <= GABE_BLOCK ? 2 : 1; }
+/* complex slots of one buffer: row_planes x 6 pairs x rows per workgroup */
 template <typename T>
-__host__ __device__ constexpr int rbuf_size(int N) { return row_planes<T>() * 6 * (GABE_BLOCK / N) * rseq_stride(N); }
+__host__ __device__ constexpr int rbuf_size(int N) { return row_planes<T>(N) * 6 * (row_block(N) / N) * rseq_stride(N); }
 
 template <typename T, int N>
 __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, typename gabe_fft::cplx_t<T>::type *zo,
@@ -97,11 +108,12 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     using C = typename gabe_fft::cplx_t<T>::type;
     constexpr int N1 = N == 16 ? 4 : (N <= 64 ? 8 : 16);
     constexpr int N2 = N / N1;
-    constexpr int R = GABE_BLOCK / N, SPP = 6 * R, H = N / 2 + 1;   /* sequences per plane */
+    constexpr int BLK = row_block(N);
+    constexpr int R = BLK / N, SPP = 6 * R, H = N / 2 + 1;   /* sequences per plane */
     constexpr int SS = rseq_stride(N);
     const int nseq = SPP * npl;
     const C *tw = (const C *)A.tw;
-    for (int task = threadIdx.x; task < nseq * N2; task += GABE_BLOCK) {
+    for (int task = threadIdx.x; task < nseq * N2; task += BLK) {
         const int sq = task / N2, n2 = task - sq * N2;
         C *b = zl + sq * SS;
         C x[N1];
@@ -116,7 +128,7 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
         }
     }
     __syncthreads();
-    for (int task = threadIdx.x; task < nseq * N1; task += GABE_BLOCK) {
+    for (int task = threadIdx.x; task < nseq * N1; task += BLK) {
         const int sq = task / N1, k1 = task - sq * N1;
         const C *b = zl + sq * SS;
         C *o = zo + sq * SS;
@@ -131,7 +143,7 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     const long comp = (long)N * N * A.kpad;
     C *out = (C *)A.shat;
     const T h = (T)0.5;
-    for (int task = threadIdx.x; task < nseq * H; task += GABE_BLOCK) {
+    for (int task = threadIdx.x; task < nseq * H; task += BLK) {
         const int sq = task / H, m = task - sq * H;
         const int pl = sq / SPP, ps = sq - pl * SPP, pr = ps / R, r = ps - pr * R;
         const C *o = zo + sq * SS;
@@ -168,7 +180,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             by = logical / gx;
         }
     }
-    const long p = (long)bx * GABE_BLOCK + threadIdx.x;
+    constexpr int BLK = ROWN > 0 ? row_block(ROWN) : GABE_BLOCK;
+    const long p = (long)bx * BLK + threadIdx.x;
     const bool valid = p < N2;
     const long pp = valid ? p : 0;
     const int j = (int)(pp / N), k = (int)(pp - (long)j * N);
@@ -309,7 +322,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                         if constexpr (ROWN > 0) {
                             /* sequence (plane, pair, row) = ((i - i0) mod row_planes, 3 st + q, t / N) */
                             using C = typename gabe_fft::cplx_t<T>::type;
-                            constexpr int RR = GABE_BLOCK / ROWN, SS = rseq_stride(ROWN), PPC = row_planes<T>();
+                            constexpr int RR = BLK / ROWN, SS = rseq_stride(ROWN), PPC = row_planes<T>(ROWN);
                             const int t = threadIdx.x, r = t / ROWN, kk = t - r * ROWN;
                             C *z = zl + ((((i - i0) % PPC) * 6 + 3 * st) * RR + r) * SS + rpad(kk);
                             C z0, z1, z2;
@@ -333,11 +346,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             }
         }
         if constexpr (ROWN > 0) {
-            constexpr int PPC = row_planes<T>();
+            constexpr int PPC = row_planes<T>(ROWN);
             if ((i - i0) % PPC == PPC - 1 || i + 1 == i1) {
                 const int npl = (i - i0) % PPC + 1;
                 __syncthreads();
-                row_r2c<T, ROWN>(zl, zo, A, i + 1 - npl, bx * (GABE_BLOCK / ROWN), npl);
+                row_r2c<T, ROWN>(zl, zo, A, i + 1 - npl, bx * (BLK / ROWN), npl);
                 __syncthreads();
             }
         }
@@ -347,12 +360,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             vm[f] = vc[f]; vc[f] = vp[f];
         }
     }
-    if (MODE != SW_B) block_store_partials<NS>(acc, A.partials, A.nblk, bid);
+    if (MODE != SW_B) block_store_partials<NS, BLK / GABE_BLOCK>(acc, A.partials, A.nblk, bid);
 }
 
-/* sweep A with the stress rows transformed in the workgroup (N 16..256) */
+/* sweep A with the stress rows transformed in the workgroup (N 16..512) */
 template <typename T, int NF, int ROWN>
-__global__ __launch_bounds__(GABE_BLOCK) void k_sweep_rowfft(SweepArgs<T> A)
+__global__ __launch_bounds__(row_block(ROWN)) void k_sweep_rowfft(SweepArgs<T> A)
 {
     using C = typename gabe_fft::cplx_t<T>::type;
     __shared__ C zl[rbuf_size<T>(ROWN)], zo[rbuf_size<T>(ROWN)];
@@ -864,6 +877,14 @@ static bool launch_rowfft(const SweepArgs<T> &a, dim3 grid, hipStream_t s)
     case 64: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 64>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
     case 128: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 128>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
     case 256: hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 256>), grid, dim3(GABE_BLOCK), 0, s, a); return true;
+    case 512:
+        /* one 512-thread workgroup per pair of the 256-thread row blocks */
+        if constexpr (sizeof(T) == 4) {
+            if (grid.x % 2) return false;
+            hipLaunchKernelGGL((k_sweep_rowfft<T, NF, 512>), dim3(grid.x / 2, grid.y), dim3(512), 0, s, a);
+            return true;
+        }
+        return false;
     default: return false;
     }
 }

@@ -286,9 +286,13 @@ def test_fused_gw_transform_matches_separate_passes(precision, N, monkeypatch):
 
 
 def test_fused_gw_transform_matches_separate_passes_512():
-    """FP32 at 512^3 (library k rows, fused i transform + modes): the same
-    tensor modes as the separate i pass and mode kernel, on the reference's
-    own initial conditions generated on the device."""
+    """FP32 at 512^3: k rows transformed in sweep A (512-thread workgroups)
+    and the fused i transform + modes give the tensor modes of the library
+    rows with the separate i pass and mode kernel, on the reference's own
+    initial conditions generated on the device.  The fields agree to FP32
+    rounding rather than bitwise: a 512-thread workgroup adds its eight waves'
+    energy sums before the finalize, where two 256-thread workgroups leave two
+    partials, so the FP64 sums (and a) differ in the last bits."""
     from gabe_amd import host
     cfg = configs.lambda_phi4_config(N=512, calc_gws=1)
     fused = host.initialized_lattice(cfg, precision=GABE_FP32)
@@ -304,7 +308,9 @@ def test_fused_gw_transform_matches_separate_passes_512():
     assert rel_err(lf, ls) < 1e-6 and rel_err(hf, hs) < 1e-6
     ff, _ = fused.download_state()
     fs, _ = sep.download_state()
-    assert np.array_equal(ff, fs)
+    for f in range(ff.shape[0]):
+        fl = fs[f] - fs[f].mean()
+        assert np.linalg.norm(ff[f] - fs[f]) <= 1e-6 * np.linalg.norm(fl)
     fused.close()
     sep.close()
 
