@@ -303,7 +303,7 @@ def main():
                                      **precision_report(fin32, fin64))}
         del fin64
 
-    ab = alg_bytes_per_site(4, N, rows_in_sweep=16 <= N <= 256 and (N & (N - 1)) == 0)
+    ab = alg_bytes_per_site(4, N, rows_in_sweep=16 <= N <= 512 and (N & (N - 1)) == 0)
     roofline = None
     if k32:
         dom = max((k for k in k32 if k != "finalize"), key=lambda k: k32[k])
