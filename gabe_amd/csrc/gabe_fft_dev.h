@@ -73,7 +73,12 @@ __device__ __forceinline__ void dft_reg(C (&x)[R])
             for (int k = 0; k < len / 2; k++) {
                 const C u = x[i + k];
                 C v = x[i + k + len / 2];
-                if (k != 0) {
+                if (k * (R / len) == R / 4) {
+                    /* w = -i: a swap and a sign, the same values as the product */
+                    const T vx = v.x;
+                    v.x = v.y;
+                    v.y = -vx;
+                } else if (k != 0) {
                     C w;
                     w.x = (T)tw.re[k * (R / len)];
                     w.y = (T)tw.im[k * (R / len)];
