@@ -9,8 +9,9 @@ restricted onto the coarse lattice (e.g. 128^3): the input is the restricted
 state, the target what the coarse lattice's Laplacian misses of the fine
 one, R(lap_fine f) - lap_coarse(R f).  Applied as a velocity kick after each
 plain step, a correction of this form moves each mode's frequency from the
-coarse lattice's to the fine lattice's at leading order in dt.  The plain coarse step is the fallback, and the fine run
-is the judge: spectra per bin, energies and the Friedmann residual.
+coarse lattice's to the fine lattice's at leading order in dt.  The plain coarse
+step is the fallback, and the fine run is the yardstick: spectra per bin,
+energies and the Friedmann residual.
 """
 import ctypes as C
 import dataclasses
@@ -242,12 +243,12 @@ def evaluate(cl, setup, seed, coarse_steps, threshold_factor=10.0, gws=True, log
 
 
 def main(argv=None):
-    """Train a closure on fine runs and judge it against a held-out fine run.
+    """Train a closure on fine runs and evaluate it against a held-out fine run.
 
     Defaults: 128^3 coarse lattice, 512^3 fine lattice (ratio 4, dt = dx/20
     on both), lambda phi^4 + g^2 phi^2 chi^2 model of the benchmark, training
     seeds 1 and 2, held-out seed 3, 1000 coarse steps with tensor modes in the
-    judged runs.  Prints one JSON line and writes the trained network."""
+    evaluated runs.  Prints one JSON line and writes the trained network."""
     import argparse
     import json
     import sys
@@ -262,7 +263,7 @@ def main(argv=None):
     ap.add_argument("--iters", type=int, default=2000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-seed", type=int, default=3)
-    ap.add_argument("--steps", type=int, default=1000, help="coarse steps of the judged runs")
+    ap.add_argument("--steps", type=int, default=1000, help="coarse steps of the evaluated runs")
     ap.add_argument("--threshold-factor", type=float, default=10.0,
                     help="fallback once the Friedmann residual exceeds this times the plain run's")
     ap.add_argument("--no-gw", action="store_true")
