@@ -191,22 +191,35 @@ int launch_gw_ctx(gabe_ctx *c)
     return GABE_OK;
 }
 
-/* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid).
+/* kz = N/2 plane [12][N][N] of the current shat buffer when sweep A packs the
+ * rows (after the 12 row blocks), else null */
+void *nyq_plane(gabe_ctx *c)
+{
+    return c->rowfft ? (char *)c->shat + 2 * c->esz * 12 * c->N2 * c->kpad : nullptr;
+}
+
+/* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid,
+ * or kz < N/2 and the kz = N/2 plane when sweep A packs the rows).
  * with_i = false stops after the j columns (the fused kernel does i). */
 int stress_fft(gabe_ctx *c, bool with_i = true, hipStream_t st = nullptr)
 {
     if (!st) st = c->stream;
     if (!c->colfft) return run_plan(c, c->fft_stress, c->stress, c->shat);
     if (!c->rowfft) CHK(run_plan(c, c->fft_stress, c->stress, c->shat));   /* rows along k */
-    const long N = c->N, KP = N / 2 + 1, kp = c->kpad;
-    ColFftView vj{c->shat, kp, N * kp, 0, (int)KP, 1, 12 * N};        /* along j */
-    ColFftView vi{c->shat, N * kp, N * N * kp, kp, (int)KP, (int)N, 12}; /* along i */
+    const long N = c->N, kp = c->kpad;
+    const int cols = c->rowfft ? (int)(N / 2) : (int)(N / 2 + 1);
+    void *nyq = nyq_plane(c);
+    ColFftView vj{c->shat, kp, N * kp, 0, cols, 1, 12 * N, nyq};       /* along j */
+    ColFftView vi{c->shat, N * kp, N * N * kp, kp, cols, (int)N, 12};  /* along i */
+    ColFftView vn{nyq, N, N * N, 0, (int)N, 1, 12};                    /* kz = N/2 plane along i */
     if (c->fp32) {
         launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, st);
         if (with_i) launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, st);
+        if (with_i && nyq) launch_colfft<float>(vn, c->N, c->fft_cw, c->twiddle, st);
     } else {
         launch_colfft<double>(vj, c->N, c->fft_cw, c->twiddle, st);
         if (with_i) launch_colfft<double>(vi, c->N, c->fft_cw, c->twiddle, st);
+        if (with_i && nyq) launch_colfft<double>(vn, c->N, c->fft_cw, c->twiddle, st);
     }
     HIPCHK(hipGetLastError());
     return GABE_OK;
@@ -220,11 +233,12 @@ int gw_modes(gabe_ctx *c, hipStream_t st = nullptr)
     const double dx3 = c->dx * c->dx * c->dx;
     if (c->fp32) {
         GwFftArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, (const float *)c->twiddle,
-                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4};
+                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4, (const float *)nyq_plane(c)};
         launch_gw_ifft<float>(g, st);
     } else {
         GwFftArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss,
-                            (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4};
+                            (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4,
+                            (const double *)nyq_plane(c)};
         launch_gw_ifft<double>(g, st);
     }
     HIPCHK(hipGetLastError());
@@ -379,12 +393,18 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     }
     const int narr = c->alloc_gw ? 12 : c->nf;
     c->colfft = c->alloc_gw && gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
-    c->kpad = c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1;
     c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
     /* k rows in sweep A: up to GABE_BLOCK sites per row in both forms; 512-site
      * rows (one 512-thread workgroup each) in FP32 */
     c->rowfft = c->gwfft && c->N >= 16 && (c->N <= GABE_BLOCK || (c->N == 512 && c->fp32)) &&
                 !std::getenv("GABE_LIBRARY_ROWS");
+    /* Row length of shat.  Sweep A's rows are N/2 slots, kz = 0 and N/2 packed
+     * into slot 0, and the j pass moves kz = N/2 into a plane of its own after
+     * the 12 row blocks (N^2 (N/2) + N^2 = M slots per component): every
+     * column chunk is full and every row a whole number of cache lines.  The
+     * library row plan writes N/2+1 values, padded to 16 for the column
+     * passes. */
+    c->kpad = c->rowfft ? c->N / 2 : (c->colfft ? ((c->N / 2 + 1 + 15) / 16) * 16 : c->N / 2 + 1);
     c->lay.N = c->N;
     c->lay.KP = c->N / 2 + 1;
     c->lay.nt4 = (int)((c->lay.KP + 3) / 4);
@@ -445,8 +465,9 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
 
     if (c->alloc_gw) {
         if (c->colfft) {
-            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12 * (size_t)c->N2, 1,
-                          (size_t)c->kpad));
+            if (!c->rowfft)
+                CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12 * (size_t)c->N2, 1,
+                              (size_t)c->kpad));
             std::vector<double> tw(2 * c->N);
             for (int e = 0; e < c->N; e++) {
                 tw[2 * e] = gabe_cos2pi_frac(e, c->N);
@@ -1043,8 +1064,12 @@ int gabe_tt_source(gabe_ctx *c, int s, double *out)
     if (rc == GABE_OK) rc = stress_fft(c);
     if (rc == GABE_OK) {
         const double dx3 = c->dx * c->dx * c->dx;
-        if (c->fp32) launch_tt_out<float>((const float *)c->shat, s, c->N, c->M, c->kpad, dx3, dout, c->stream);
-        else launch_tt_out<double>((const double *)c->shat, s, c->N, c->M, c->kpad, dx3, dout, c->stream);
+        if (c->fp32)
+            launch_tt_out<float>((const float *)c->shat, (const float *)nyq_plane(c), s, c->N, c->M, c->kpad, dx3,
+                                 dout, c->stream);
+        else
+            launch_tt_out<double>((const double *)c->shat, (const double *)nyq_plane(c), s, c->N, c->M, c->kpad,
+                                  dx3, dout, c->stream);
         if (hipGetLastError() != hipSuccess) rc = fail(GABE_ERR_HIP, "tt kernel launch failed");
     }
     if (rc == GABE_OK && hipMemcpyAsync(out, dout, sizeof(double) * 12 * c->M, hipMemcpyDeviceToHost, c->stream) != hipSuccess)

@@ -33,20 +33,23 @@ constexpr int kColThreads = 256;
 template <typename T, int N1, int N2, int CW>
 __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type *data, long axis_stride,
                                                        long outer_stride, long sub_stride, int sub_valid,
-                                                       int chunks_per_sub, const typename cplx_t<T>::type *tw)
+                                                       int chunks_per_sub, const typename cplx_t<T>::type *tw,
+                                                       typename cplx_t<T>::type *nyq)
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2;
     constexpr int PER = (N * CW + kColThreads - 1) / kColThreads;   /* loads per thread */
     constexpr int BATCH = PER < 16 ? PER : 16;
     static_assert(PER % BATCH == 0, "tile must split into whole batches");
-    static_assert(sizeof(C) * N * (CW + 1) <= 160 * 1024, "tile exceeds the LDS of one CU");
+    static_assert(sizeof(C) * N * (CW + 2) <= 160 * 1024, "tile exceeds the LDS of one CU");
     __shared__ C tile[N][CW + 1];
+    __shared__ C packed[N];   /* transformed column 0 when it holds a packed pair */
     /* XCD-aware order: neighbouring column chunks on one XCD (one L2) */
     int b = blockIdx.x;
     if (gridDim.x % 8 == 0) b = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
     const int sidx = b / chunks_per_sub, q = b % chunks_per_sub;
     const int width = min(CW, sub_valid - q * CW);
+    const bool split = nyq != nullptr && sidx == 0 && q == 0;   /* uniform over the workgroup */
     C *base = data + (long)blockIdx.y * outer_stride + (long)sidx * sub_stride + (long)q * CW;
 
 #pragma unroll
@@ -87,9 +90,27 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
 #pragma unroll
         for (int n2 = 0; n2 < N2; n2++) y[n2] = tile[N2 * k1 + n2][cc];
         dft_reg<N2, T>(y);
-        if (cc < width) {
+        if (split && cc == 0) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) packed[k1 + N1 * k2] = y[k2];
+        } else if (cc < width) {
 #pragma unroll
             for (int k2 = 0; k2 < N2; k2++) base[(long)(k1 + N1 * k2) * axis_stride + cc] = y[k2];
+        }
+    }
+    if (split) {
+        __syncthreads();
+        C *ny = nyq + (long)blockIdx.y * N;
+        const T h = (T)0.5;
+        for (int n = threadIdx.x; n < N; n += kColThreads) {
+            const C Z = packed[n], W = packed[(N - n) & (N - 1)];
+            C x, y;
+            x.x = h * (Z.x + W.x);
+            x.y = h * (Z.y - W.y);
+            y.x = h * (Z.y + W.y);
+            y.y = -h * (Z.x - W.x);
+            base[(long)n * axis_stride] = x;
+            ny[n] = y;
         }
     }
 }
@@ -101,17 +122,17 @@ void launch_cw(const ColFftView &v, const void *tw, hipStream_t s)
     const int chunks = (v.sub_valid + CW - 1) / CW;
     const dim3 grid((unsigned)(v.nsub * chunks), (unsigned)v.outer);
     hipLaunchKernelGGL((k_colfft<T, N1, N2, CW>), grid, dim3(kColThreads), 0, s, (C *)v.data, v.axis_stride,
-                       v.outer_stride, v.sub_stride, v.sub_valid, chunks, (const C *)tw);
+                       v.outer_stride, v.sub_stride, v.sub_valid, chunks, (const C *)tw, (C *)v.nyq);
 }
 
 template <typename T, int N1, int N2>
 void launch_n(const ColFftView &v, int cw, const void *tw, hipStream_t s)
 {
     constexpr size_t csz = 2 * sizeof(T);
-    if constexpr (csz * N1 * N2 * 33 <= 160 * 1024) {
+    if constexpr (csz * N1 * N2 * 34 <= 160 * 1024) {
         if (cw == 32) { launch_cw<T, N1, N2, 32>(v, tw, s); return; }
     }
-    if constexpr (csz * N1 * N2 * 17 <= 160 * 1024) launch_cw<T, N1, N2, 16>(v, tw, s);
+    if constexpr (csz * N1 * N2 * 18 <= 160 * 1024) launch_cw<T, N1, N2, 16>(v, tw, s);
     else launch_cw<T, N1, N2, 8>(v, tw, s);
 }
 

@@ -19,6 +19,8 @@ struct GwFftArgs {
     long M, kpad;
     double L, dt, dx3;
     int nt4;            /* kz tiles of 4 of the tiled mode order */
+    const T *nyq;       /* null: rows hold kz 0..N/2.  Else rows hold kz < N/2
+                         * (kpad = N/2) and nyq [12][N][N] the kz = N/2 plane */
 };
 
 /* whether the fused kernel exists for this size and storage type: N a power

@@ -3,7 +3,9 @@
  * update.
  *
  * After the k-row R2C and the j-column transform, one workgroup takes the
- * N x CW tile of modes (all i, CW adjacent kz) at one j.  It transforms the 12
+ * N x CW tile of modes (all i, CW adjacent kz) at one j, or, when the kz = N/2
+ * values sit in a plane of their own, all i at CW adjacent j of that plane.
+ * It transforms the 12
  * stress components along i in LDS, CPR components per round.  After each
  * round, every thread projects the freshly transformed components of its own
  * modes onto the two TT polarisations and accumulates them.  Once all 12 are
@@ -183,11 +185,31 @@ void k_gw_ifft(GwFftArgs<T> A)
      * on one XCD (one L2), working at the same time */
     int b = blockIdx.x;
     if (gridDim.x % 8 == 0) b = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
-    const int q = b % NCH, j = b / NCH;
-    const int width = min(CW, KP - q * CW);
-    const int py = j <= N / 2 ? j : j - N;
-    const long SM = (long)N * N * A.kpad, rstride = (long)N * A.kpad;
-    const C *Sh = reinterpret_cast<const C *>(A.shat) + ((long)j * A.kpad + (long)q * CW);
+    /* Workgroup b < N nch: kz chunk q at one j.  With a separate kz = N/2
+     * plane, the last N / CW workgroups take CW adjacent j of that plane.
+     * Column col of the tile is the mode (j0 + col jstep, pz0 + col pzstep). */
+    const int nch = A.nyq ? N / 2 / CW : NCH;
+    const bool plane = b >= N * nch;
+    int q, j0, jstep, pz0, pzstep, width;
+    long SM, rstride;
+    const C *Sh;
+    if (!plane) {
+        q = b % nch;
+        j0 = b / nch; jstep = 0;
+        pz0 = q * CW; pzstep = 1;
+        width = min(CW, KP - q * CW);
+        SM = (long)N * N * A.kpad;
+        rstride = (long)N * A.kpad;
+        Sh = reinterpret_cast<const C *>(A.shat) + ((long)j0 * A.kpad + (long)q * CW);
+    } else {
+        q = b - N * nch;
+        j0 = q * CW; jstep = 1;
+        pz0 = N / 2; pzstep = 0;
+        width = CW;
+        SM = (long)N * N;
+        rstride = N;
+        Sh = reinterpret_cast<const C *>(A.nyq) + (long)q * CW;
+    }
 
     /* twiddles exp(-2 pi i e/N) */
     for (int e = threadIdx.x; e < N; e += NT) twl[e] = reinterpret_cast<const C *>(A.tw)[e];
@@ -198,7 +220,8 @@ void k_gw_ifft(GwFftArgs<T> A)
 #pragma unroll
     for (int u = 0; u < MPT; u++) {
         const int e = threadIdx.x + u * NT, i = e / CW, col = e % CW;
-        const int px = i <= N / 2 ? i : i - N, pz = q * CW + col;
+        const int jj = j0 + col * jstep, py = jj <= N / 2 ? jj : jj - N;
+        const int px = i <= N / 2 ? i : i - N, pz = pz0 + col * pzstep;
         const bool has = e < N * CW && col < width && (px != 0 || py != 0 || pz != 0);
         double du[3] = {0., 0., 0.}, dv[3] = {0., 0., 0.};
         if (has) pol_uv(px, py, pz, du, dv);
@@ -225,11 +248,12 @@ void k_gw_ifft(GwFftArgs<T> A)
     for (int u = 0; u < MPT; u++) {
         const int e = threadIdx.x + u * NT, i = e / CW, col = e % CW;
         if (e >= N * CW || col >= width) continue;
-        const int px = i <= N / 2 ? i : i - N, pz = q * CW + col;
+        const int jj = j0 + col * jstep, py = jj <= N / 2 ? jj : jj - N;
+        const int px = i <= N / 2 ? i : i - N, pz = pz0 + col * pzstep;
         const double p2 = (double)(px * px + py * py + pz * pz);
         const T kk0 = (T)(k2u * p2 * ia0), kk1 = (T)(k2u * p2 * ia1);
         /* tiled mode order [j][kz/4][i][kz%4] */
-        const long m = (((long)j * A.nt4 + (pz >> 2)) * N + i) * 4 + (pz & 3);
+        const long m = (((long)jj * A.nt4 + (pz >> 2)) * N + i) * 4 + (pz & 3);
 #pragma unroll
         for (int pol = 0; pol < 2; pol++) {
             const C h0 = hh[2 * m + pol], l0 = ll[2 * m + pol];
@@ -253,7 +277,9 @@ template <typename T, int N1, int N2, int CW, int CPR, int NT>
 void launch_cfg(const GwFftArgs<T> &a, hipStream_t s)
 {
     constexpr int N = N1 * N2, NCH = (N / 2 + 1 + CW - 1) / CW;
-    hipLaunchKernelGGL((k_gw_ifft<T, N1, N2, CW, CPR, NT>), dim3(N * NCH), dim3(NT), 0, s, a);
+    static_assert((N / 2) % CW == 0 && N % CW == 0, "kz chunks and plane chunks must be whole");
+    const int grid = a.nyq ? N * (N / 2 / CW) + N / CW : N * NCH;
+    hipLaunchKernelGGL((k_gw_ifft<T, N1, N2, CW, CPR, NT>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 } // namespace

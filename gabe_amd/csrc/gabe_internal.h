@@ -55,7 +55,8 @@ struct SweepArgs {
     T *fo[GABE_MAXF];       /* sweep B output state */
     T *vo[GABE_MAXF];
     T *stress;              /* [12][N^3]: S(f0) comps 0-5, S(f1) comps 6-11 */
-    T *shat;                /* [12][N][N][kpad] complex: row transforms written by the fused sweep A */
+    T *shat;                /* [12][N][N][kpad = N/2] complex: row transforms written by the fused sweep A,
+                             * slot 0 = S(kz 0) + i S(kz N/2) */
     const T *tw;            /* exp(-2 pi i e / N), e < N, complex */
     long kpad;
     int rowfft;             /* N when sweep A transforms its k-rows itself, else 0 */
@@ -121,7 +122,8 @@ void launch_gw_pack(const double *ref, T *pol, ModeLayout lay, long M, hipStream
 template <typename T>
 void launch_gw_unpack(const T *pol, double *ref, ModeLayout lay, long M, hipStream_t s);
 template <typename T>
-void launch_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3, double *out, hipStream_t s);
+void launch_tt_out(const T *shat, const T *nyq, int stage, int N, long M, long kpad, double dx3, double *out,
+                   hipStream_t s);
 template <typename T>
 void launch_spectra(const SpecArgs<T> &a, hipStream_t s);
 template <typename T>

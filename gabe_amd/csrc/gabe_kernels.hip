@@ -81,8 +81,10 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
  * transform of length N per pair (two-step N = N1 N2, in-register DFTs) and
  * the split
  *   S_c^(m) = (Z(m) + conj Z(N-m)) / 2,  S_(c+1)^(m) = (Z(m) - conj Z(N-m)) / 2i
- * give both real rows' transforms, m = 0..N/2.  Two planes per call halve
- * the barriers per plane, which stall the sweep's loads. */
+ * give both real rows' transforms, m = 0..N/2.  Rows are stored N/2 slots
+ * long: S(0) and S(N/2), both real, share slot 0 as S(0) + i S(N/2) (the
+ * j-column pass separates them).  Two planes per call halve the barriers per
+ * plane, which stall the sweep's loads. */
 /* LDS layout of the row sequences: element n of sequence sq at
  * sq * rseq_stride(N) + rpad(n), one padding slot every 16 elements, so the
  * stride-16 reads of the second DFT step and the sequences' starts fall on
@@ -109,7 +111,7 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     constexpr int N1 = N == 16 ? 4 : (N <= 64 ? 8 : 16);
     constexpr int N2 = N / N1;
     constexpr int BLK = row_block(N);
-    constexpr int R = BLK / N, SPP = 6 * R, H = N / 2 + 1;   /* sequences per plane */
+    constexpr int R = BLK / N, SPP = 6 * R, H = N / 2;   /* sequences per plane, slots per row */
     constexpr int SS = rseq_stride(N);
     const int nseq = SPP * npl;
     const C *tw = (const C *)A.tw;
@@ -147,13 +149,19 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
         const int sq = task / H, m = task - sq * H;
         const int pl = sq / SPP, ps = sq - pl * SPP, pr = ps / R, r = ps - pr * R;
         const C *o = zo + sq * SS;
-        const C Z = o[rpad(m)], W = o[rpad(m == 0 ? 0 : N - m)];
+        /* m = 0 packs both real values: slot 0 = S(0) + i S(N/2) */
+        const C Z = o[rpad(m)], W = o[rpad(m == 0 ? N / 2 : N - m)];
         const long row = ((long)(i + pl) * N + (j0 + r)) * A.kpad + m;
         C s0, s1;
-        s0.x = h * (Z.x + W.x);
-        s0.y = h * (Z.y - W.y);
-        s1.x = h * (Z.y + W.y);
-        s1.y = -h * (Z.x - W.x);
+        if (m == 0) {
+            s0.x = Z.x; s0.y = W.x;
+            s1.x = Z.y; s1.y = W.y;
+        } else {
+            s0.x = h * (Z.x + W.x);
+            s0.y = h * (Z.y - W.y);
+            s1.x = h * (Z.y + W.y);
+            s1.y = -h * (Z.x - W.x);
+        }
         out[(2 * pr) * comp + row] = s0;
         out[(2 * pr + 1) * comp + row] = s1;
     }
@@ -706,10 +714,12 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw_unpack(const T *pol, double *
     }
 }
 
-/* T_gw[s] in the reference layout [12][M] (diagnostic, not on the step path) */
+/* T_gw[s] in the reference layout [12][M] (diagnostic, not on the step path).
+ * With nyq set, shat rows hold kz < N/2 and the kz = N/2 plane is
+ * nyq [12][N][N]. */
 template <typename T>
-__global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3,
-                                                         double *out)
+__global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, const T *nyq, int stage, int N, long M, long kpad,
+                                                         double dx3, double *out)
 {
     using C = typename cplx_of<T>::type;
     const long m = (long)blockIdx.x * GABE_BLOCK + threadIdx.x;
@@ -722,9 +732,12 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, int stage,
     const int p2 = px * px + py * py + pz * pz;
     const long SM = (long)N * N * kpad;
     const C *Sh = reinterpret_cast<const C *>(shat) + (long)(6 * stage) * SM;
+    const bool plane = nyq != nullptr && kz == N / 2;
+    const long NN = (long)N * N;
+    const C *Sn = reinterpret_cast<const C *>(nyq) + (long)(6 * stage) * NN;
     C S[6], Tt[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) S[c] = Sh[(long)c * SM + r * kpad + kz];
+    for (int c = 0; c < 6; c++) S[c] = plane ? Sn[(long)c * NN + r] : Sh[(long)c * SM + r * kpad + kz];
     if (p2 == 0) {
 #pragma unroll
         for (int c = 0; c < 6; c++) Tt[c].x = Tt[c].y = 0;
@@ -959,10 +972,10 @@ void launch_gw_unpack(const T *pol, double *ref, ModeLayout lay, long M, hipStre
 }
 
 template <typename T>
-void launch_tt_out(const T *shat, int stage, int N, long M, long kpad, double dx3, double *out, hipStream_t s)
+void launch_tt_out(const T *shat, const T *nyq, int stage, int N, long M, long kpad, double dx3, double *out, hipStream_t s)
 {
     const unsigned nb = (unsigned)((M + GABE_BLOCK - 1) / GABE_BLOCK);
-    hipLaunchKernelGGL((k_tt_out<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, shat, stage, N, M, kpad, dx3, out);
+    hipLaunchKernelGGL((k_tt_out<T>), dim3(nb), dim3(GABE_BLOCK), 0, s, shat, nyq, stage, N, M, kpad, dx3, out);
 }
 
 template <typename T>
@@ -1001,7 +1014,7 @@ void launch_add_real(T *dst, const double *add, long n, hipStream_t s)
     template void launch_gw<T>(const GwArgs<T> &, hipStream_t);                                               \
     template void launch_gw_pack<T>(const double *, T *, ModeLayout, long, hipStream_t);                     \
     template void launch_gw_unpack<T>(const T *, double *, ModeLayout, long, hipStream_t);                   \
-    template void launch_tt_out<T>(const T *, int, int, long, long, double, double *, hipStream_t);           \
+    template void launch_tt_out<T>(const T *, const T *, int, int, long, long, double, double *, hipStream_t); \
     template void launch_spectra<T>(const SpecArgs<T> &, hipStream_t);                                        \
     template void launch_effmass<T>(const T *, const T *, int, int, double, double, gabe_model, long,        \
                                     double *, int, hipStream_t);                                              \
