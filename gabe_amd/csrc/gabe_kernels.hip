@@ -167,11 +167,148 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     }
 }
 
+/* FP32 sweep B: the plane march of sweep_body for the new state, with the
+ * centres of plane i + 2 loaded while plane i is evaluated.  Four plane
+ * buffers rotate: plane i reads planes i - 1, i, i + 1 from three of them and
+ * loads plane i + 2 into the fourth, whose plane i - 2 is no longer needed.
+ * The march is unrolled by four with the buffer roles fixed at compile time,
+ * so no register is copied (a copy out of a register still being loaded
+ * would wait for the load).  The loads are issued after the plane's
+ * neighbour loads, so the in-order waits for those leave them in flight
+ * through the force evaluation and the stores.  (Loading further ahead
+ * gains nothing: the wait for the next plane's neighbour loads also waits
+ * for every load issued before them.)  The arithmetic is sweep_body's,
+ * expression for expression. */
+template <int Q>
+using plane_role = std::integral_constant<int, Q>;
+
+template <typename T, int NF>
+__device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int gx, int gy)
+{
+    const int N = A.N;
+    const long N2 = A.N2;
+    int bx = hw % gx, by = hw / gx;
+    {
+        const int nb = gx * gy;
+        if (nb % GABE_NXCD == 0) {
+            const int logical = (hw % GABE_NXCD) * (nb / GABE_NXCD) + hw / GABE_NXCD;
+            bx = logical % gx;
+            by = logical / gx;
+        }
+    }
+    const long p = (long)bx * GABE_BLOCK + threadIdx.x;
+    const bool valid = p < N2;
+    const long pp = valid ? p : 0;
+    const int j = (int)(pp / N), k = (int)(pp - (long)j * N);
+    /* byte offsets within a plane (N^2 sites of T fit 32 bits): each load is
+     * a uniform plane base plus a 32-bit lane offset */
+    const unsigned ocen = (unsigned)pp * sizeof(T);
+    const unsigned okp = (unsigned)((long)j * N + (k + 1 == N ? 0 : k + 1)) * sizeof(T);
+    const unsigned okm = (unsigned)((long)j * N + (k == 0 ? N - 1 : k - 1)) * sizeof(T);
+    const unsigned ojp = (unsigned)((long)(j + 1 == N ? 0 : j + 1) * N + k) * sizeof(T);
+    const unsigned ojm = (unsigned)((long)(j == 0 ? N - 1 : j - 1) * N + k) * sizeof(T);
+    auto at = [&](const T *arr, int plane_i, unsigned off) -> T {
+        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(arr + (long)plane_i * N2) + off);
+    };
+    const int i0 = by * A.ci;
+    const int i1 = min(i0 + A.ci, N);
+
+    const T idx2 = (T)A.inv_dx2;
+    const double dt = A.dt, hdt = 0.5 * dt;
+    const T hdtT = (T)hdt;
+    const StepScalars *ss = A.ss;
+    const double ia02 = 1. / (ss->a0 * ss->a0), ia12 = 1. / (ss->a1 * ss->a1);
+    const double H0 = ss->adot0 / ss->a0, H1 = ss->adot1 / ss->a1;
+    double pb0[2] = {0., 0.}, pb1[2] = {0., 0.}, mvp0[2] = {0., 0.}, mvp1[2] = {0., 0.};
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        pb0[f] = ss->pbar0[f]; pb1[f] = ss->pbar1[f];
+        mvp0[f] = ss->meanVp0[f]; mvp1[f] = ss->meanVp1[f];
+    }
+
+    /* centre values of four consecutive planes, field and velocity */
+    T bf[4][NF], bv[4][NF];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int ic = (i0 - 1 + q + N) % N;   /* planes i0 - 1 .. i0 + 1 */
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            bf[q][f] = at(A.f[f], ic, ocen);
+            bv[q][f] = at(A.v[f], ic, ocen);
+        }
+    }
+
+    /* plane i with planes i - 1, i, i + 1 in buffers Q, Q + 1, Q + 2 (mod 4);
+     * plane i + 2 goes into buffer Q + 3 */
+    auto plane = [&](auto role, const int i) {
+        constexpr int Q = decltype(role)::value;
+        constexpr int QM = Q % 4, QC = (Q + 1) % 4, QP = (Q + 2) % 4, QN = (Q + 3) % 4;
+        T lap0[NF], lap1[NF];
+        double full0[2] = {0., 0.}, full1[2] = {0., 0.};
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const T fm = bf[QM][f], fc = bf[QC][f], fp = bf[QP][f];
+            const T vm = bv[QM][f], vc = bv[QC][f], vp = bv[QP][f];
+            const T fkp = at(A.f[f], i, okp), fkm = at(A.f[f], i, okm);
+            const T fjp = at(A.f[f], i, ojp), fjm = at(A.f[f], i, ojm);
+            lap0[f] = (((fm + fp) + (fjm + fjp)) + (fkm + fkp) - (T)6 * fc) * idx2;
+            full0[f] = pb0[f] + (double)fc;
+            const T vkp = at(A.v[f], i, okp), vkm = at(A.v[f], i, okm);
+            const T vjp = at(A.v[f], i, ojp), vjm = at(A.v[f], i, ojm);
+            const T m1 = midpt(fm, vm, hdtT), p1 = midpt(fp, vp, hdtT);
+            const T jm1 = midpt(fjm, vjm, hdtT), jp1 = midpt(fjp, vjp, hdtT);
+            const T km1 = midpt(fkm, vkm, hdtT), kp1 = midpt(fkp, vkp, hdtT);
+            const T c1 = midpt(fc, vc, hdtT);
+            lap1[f] = (((m1 + p1) + (jm1 + jp1)) + (km1 + kp1) - (T)6 * c1) * idx2;
+            full1[f] = pb1[f] + (double)c1;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int i2 = i + 2 < N ? i + 2 : i + 2 - N;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            bf[QN][f] = at(A.f[f], i2, ocen);
+            bv[QN][f] = at(A.v[f], i2, ocen);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (valid) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const double Vp0 = gabe_dVdf(A.model, f, full0[0], full0[1]);
+                const double Vp1 = gabe_dVdf(A.model, f, full1[0], full1[1]);
+                const double dv0 = (double)bv[QC][f];
+                /* F(0) and F(1) of ddfield (g2functions.cpp:193-196) minus the
+                 * background force */
+                const double dF0 = (double)lap0[f] * ia02 - (Vp0 - mvp0[f]) - 3. * H0 * dv0;
+                const double dv1 = dv0 + hdt * dF0;
+                const double dF1 = (double)lap1[f] * ia12 - (Vp1 - mvp1[f]) - 3. * H1 * dv1;
+                *reinterpret_cast<T *>(reinterpret_cast<char *>(A.fo[f] + (long)i * N2) + ocen) =
+                    (T)((double)bf[QC][f] + dt * dv1);
+                *reinterpret_cast<T *>(reinterpret_cast<char *>(A.vo[f] + (long)i * N2) + ocen) =
+                    (T)(dv0 + dt * dF1);
+            }
+        }
+    };
+    int i = i0;
+    for (; i + 3 < i1; i += 4) {
+        plane(plane_role<0>{}, i);
+        plane(plane_role<1>{}, i + 1);
+        plane(plane_role<2>{}, i + 2);
+        plane(plane_role<3>{}, i + 3);
+    }
+    if (i < i1) plane(plane_role<0>{}, i);
+    if (i + 1 < i1) plane(plane_role<1>{}, i + 1);
+    if (i + 2 < i1) plane(plane_role<2>{}, i + 2);
+}
+
 template <typename T, int NF, int MODE, bool GW, int ROWN = 0>
 __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy,
                                            typename gabe_fft::cplx_t<T>::type *zl = nullptr,
                                            typename gabe_fft::cplx_t<T>::type *zo = nullptr)
 {
+    if constexpr (MODE == SW_B && sizeof(T) == 4) {
+        sweep_b_ahead<T, NF>(A, hw, gx, gy);
+        return;
+    }
     constexpr int NS = gabe_nsums(NF);
     const int N = A.N;
     const long N2 = A.N2;
