@@ -558,14 +558,53 @@ template <int NF>
 __device__ __forceinline__ void finalize_body(const FinArgs &F)
 {
     constexpr int NS = gabe_nsums(NF);
+    /* the state scalars the tail (thread 0) needs, read before the reduction
+     * so that their latency hides behind it */
+    DevScalars *sc = F.sc;
+    double a0 = 0., adot_prev = 0., t_old = 0., pbv[NF], vbv[NF];
+    long long nstep = 0;
+    int adot0_valid = 0;
+#pragma unroll
+    for (int f = 0; f < NF; f++) { pbv[f] = 0.; vbv[f] = 0.; }
+    if (threadIdx.x == 0) {
+        a0 = sc->a[0];
+        adot_prev = sc->adot[0];
+        adot0_valid = sc->adot0_valid;
+        t_old = sc->t;
+        nstep = sc->nstep;
+#pragma unroll
+        for (int f = 0; f < NF; f++) { pbv[f] = sc->pbar[f]; vbv[f] = sc->vbar[f]; }
+    }
+
+    /* each thread adds partials b, b + GABE_BLOCK, ... in that order; four of
+     * them are loaded at a time so that one wait covers 4 NS loads */
     double s[NS], c[NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) { s[q] = 0.; c[q] = 0.; }
-    for (int b = threadIdx.x; b < F.nblk; b += GABE_BLOCK) {
+    const int nb = F.nblk;
+    int b = threadIdx.x;
+    for (; b + 3 * GABE_BLOCK < nb; b += 4 * GABE_BLOCK) {
+        double v[4][NS];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int q = 0; q < NS; q++) v[u][q] = F.partials[(long)q * nb + b + u * GABE_BLOCK];
+        __builtin_amdgcn_sched_barrier(0);   /* all loads issued before the first add */
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int q = 0; q < NS; q++) {
+                double t, e;
+                two_sum(s[q], v[u][q], t, e);
+                s[q] = t;
+                c[q] += e;
+            }
+    }
+    for (; b < nb; b += GABE_BLOCK) {
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             double t, e;
-            two_sum(s[q], F.partials[(long)q * F.nblk + b], t, e);
+            two_sum(s[q], F.partials[(long)q * nb + b], t, e);
             s[q] = t;
             c[q] += e;
         }
@@ -601,9 +640,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs &F)
     __syncthreads();
     if (threadIdx.x != 0) return;
 
-    DevScalars *sc = F.sc;
     const double N3 = F.N3, dt = F.dt;
-    const double a0 = sc->a[0];
     double K0 = 0., G0 = 0.;
     for (int f = 0; f < F.nf; f++) {
         K0 += tot[2 + 7 * f + 0];
@@ -617,7 +654,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs &F)
     if (F.mode == SW_ENERGY) {
         sc->edkin[0] = edkin0; sc->edpot[0] = edpot0; sc->edgrad[0] = edgrad0; sc->edrho[0] = edrho0;
         sc->e0_valid = 1;
-        if (F.refresh_adot && !sc->adot0_valid) {
+        if (F.refresh_adot && !adot0_valid) {
             sc->adot[0] = F.exp_type == 0 ? 0. : sqrt(8. * M_PI / 3. * edrho0) * a0;
             sc->adot0_valid = 1;
         }
@@ -629,7 +666,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs &F)
      * the value set by the host for the first step (initexpansion, Q4) */
     double adot0;
     if (F.exp_type == 0) adot0 = 0.;
-    else if (sc->adot0_valid) adot0 = sc->adot[0];
+    else if (adot0_valid) adot0 = adot_prev;
     else adot0 = sqrt(8. * M_PI / 3. * edrho0) * a0;
     const double H0 = adot0 / a0;
 
@@ -653,11 +690,11 @@ __device__ __forceinline__ void finalize_body(const FinArgs &F)
 
     StepScalars *ss = F.ss;
     ss->a0 = a0; ss->adot0 = adot0; ss->a1 = a1; ss->adot1 = adot1;
-    for (int f = 0; f < F.nf; f++) {
+    for (int f = 0; f < NF; f++) {   /* NF = F.nf (the launch selects it) */
         const int b = 2 + 7 * f;
         const double mv0 = F.split ? tot[b + 5] / N3 : 0.;
         const double mv1 = F.split ? tot[b + 6] / N3 : 0.;
-        const double pb0 = sc->pbar[f], vb0 = sc->vbar[f];
+        const double pb0 = pbv[f], vb0 = vbv[f];
         const double pb1 = pb0 + .5 * dt * vb0;
         const double vb1 = vb0 + .5 * dt * (-mv0 - 3. * H0 * vb0);
         ss->pbar0[f] = pb0; ss->vbar0[f] = vb0; ss->pbar1[f] = pb1; ss->vbar1[f] = vb1;
@@ -666,16 +703,16 @@ __device__ __forceinline__ void finalize_body(const FinArgs &F)
         sc->vbar[f] = vb0 + dt * (-mv1 - 3. * H1 * vb1);
     }
     if (F.hist && F.hist_cap > 0) {
-        double *row = F.hist + (sc->nstep % F.hist_cap) * GABE_HIST_COLS;
-        row[0] = sc->t; row[1] = a0; row[2] = adot0; row[3] = edkin0;
-        row[4] = edpot0; row[5] = edgrad0; row[6] = edrho0; row[7] = (double)sc->nstep;
+        double *row = F.hist + (nstep % F.hist_cap) * GABE_HIST_COLS;
+        row[0] = t_old; row[1] = a0; row[2] = adot0; row[3] = edkin0;
+        row[4] = edpot0; row[5] = edgrad0; row[6] = edrho0; row[7] = (double)nstep;
     }
     sc->edkin[0] = edkin0; sc->edpot[0] = edpot0; sc->edgrad[0] = edgrad0; sc->edrho[0] = edrho0;
     sc->edkin[1] = edkin1; sc->edpot[1] = edpot1; sc->edgrad[1] = edgrad1; sc->edrho[1] = edrho1;
     sc->a[0] = a0n; sc->a[1] = a1;
     sc->adot[0] = adot0; sc->adot[1] = adot1;
-    sc->t += dt;
-    sc->nstep += 1;
+    sc->t = t_old + dt;
+    sc->nstep = nstep + 1;
     sc->adot0_valid = 0;
     sc->e0_valid = 0;
     if (!isfinite(edrho0) || !isfinite(edrho1)) sc->error = 1;
