@@ -2,11 +2,12 @@
  * gabe_fft.hip -- batched complex column transforms for the stress spectra.
  *
  * After the contiguous-row R2C along k, the tensor source needs complex
- * transforms along j (stride N/2+1) and along i (stride N(N/2+1)).  A generic
+ * transforms along j (stride kpad) and along i (stride N kpad).  A generic
  * library plan does these as strided column passes; here one workgroup takes
  * CW adjacent columns (CW complex contiguous per row, a full cache line in
- * FP32), stages the N x CW tile in LDS and transforms it with the two-step
- * factorisation N = N1 N2:
+ * FP32) and transforms them with the two-step factorisation N = N1 N2, the
+ * first step's inputs loaded into registers and its outputs exchanged through
+ * an N x CW tile in LDS:
  *   A[n2][k1] = W_N^(n2 k1) sum_n1 x[N2 n1 + n2] W_N1^(n1 k1)      (step 1)
  *   X[k1 + N1 k2] = sum_n2 A[n2][k1] W_N2^(n2 k2)                   (step 2)
  * Each short DFT runs in registers with compile-time twiddles; the inter-step
@@ -28,8 +29,9 @@ constexpr int kColThreads = 256;
 /* Element (o, r, s, c) of the view [outer][N][sub][cols] sits at
  * o*outer_stride + r*axis_stride + s*sub_stride + c.  Each workgroup takes CW
  * adjacent columns (c < sub_valid) of one (o, s) and transforms them along r,
- * in place.  Loads are issued in batches into registers before the LDS
- * stores so that many global reads are in flight per wave. */
+ * in place.  The first step reads its inputs straight from the columns (each
+ * task's N1 loads in flight together, CW adjacent columns per row across the
+ * lanes); only its outputs pass through LDS to the second step. */
 template <typename T, int N1, int N2, int CW>
 __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type *data, long axis_stride,
                                                        long outer_stride, long sub_stride, int sub_valid,
@@ -38,9 +40,6 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2;
-    constexpr int PER = (N * CW + kColThreads - 1) / kColThreads;   /* loads per thread */
-    constexpr int BATCH = PER < 16 ? PER : 16;
-    static_assert(PER % BATCH == 0, "tile must split into whole batches");
     static_assert(sizeof(C) * N * (CW + 2) <= 160 * 1024, "tile exceeds the LDS of one CU");
     __shared__ C tile[N][CW + 1];
     __shared__ C packed[N];   /* transformed column 0 when it holds a packed pair */
@@ -52,28 +51,14 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
     const bool split = nyq != nullptr && sidx == 0 && q == 0;   /* uniform over the workgroup */
     C *base = data + (long)blockIdx.y * outer_stride + (long)sidx * sub_stride + (long)q * CW;
 
-#pragma unroll
-    for (int b0 = 0; b0 < PER; b0 += BATCH) {
-        C v[BATCH];
-#pragma unroll
-        for (int u = 0; u < BATCH; u++) {
-            const int e = threadIdx.x + (b0 + u) * kColThreads, r = e / CW, cc = e % CW;
-            if (e < N * CW && cc < width) v[u] = base[(long)r * axis_stride + cc];
-            else v[u].x = v[u].y = (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < BATCH; u++) {
-            const int e = threadIdx.x + (b0 + u) * kColThreads, r = e / CW, cc = e % CW;
-            if (e < N * CW) tile[r][cc] = v[u];
-        }
-    }
-    __syncthreads();
-
     for (int task = threadIdx.x; task < N2 * CW; task += kColThreads) {
         const int cc = task % CW, n2 = task / CW;
         C x[N1];
 #pragma unroll
-        for (int n1 = 0; n1 < N1; n1++) x[n1] = tile[N2 * n1 + n2][cc];
+        for (int n1 = 0; n1 < N1; n1++) {
+            if (cc < width) x[n1] = base[(long)(N2 * n1 + n2) * axis_stride + cc];
+            else x[n1].x = x[n1].y = (T)0;
+        }
         dft_reg<N1, T>(x);
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
