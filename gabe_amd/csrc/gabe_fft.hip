@@ -60,10 +60,12 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
             else x[n1].x = x[n1].y = (T)0;
         }
         dft_reg<N1, T>(x);
+        /* n2 = 0 multiplies by tw[0] = (1, -0): the value unchanged (up to the
+         * sign of a zero), the lanes kept together */
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
             C v = x[k1];
-            if (k1 != 0 && n2 != 0) v = cmul(v, tw[(n2 * k1) % N]);
+            if (k1 != 0) v = cmul(v, tw[(n2 * k1) % N]);
             tile[N2 * k1 + n2][cc] = v;
         }
     }

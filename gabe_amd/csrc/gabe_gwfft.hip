@@ -100,7 +100,8 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
 #pragma unroll
                 for (int k1 = 0; k1 < N1; k1++) {
                     C w = x[k1];
-                    if (k1 != 0 && n2 != 0) w = cmul(w, twl[(n2 * k1) % N]);
+                    /* n2 = 0: twl[0] = (1, -0) leaves w unchanged up to the sign of a zero */
+                    if (k1 != 0) w = cmul(w, twl[(n2 * k1) % N]);
                     tile[r][trow2<N2>(k1, n2)][col] = w;
                 }
             }

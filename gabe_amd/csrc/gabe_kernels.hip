@@ -122,10 +122,12 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) x[n1] = b[rpad(N2 * n1 + n2)];
         gabe_fft::dft_reg<N1, T>(x);
+        /* lanes with n2 = 0 multiply by tw[0] = (1, -0), which returns the value
+         * unchanged (up to the sign of a zero) and keeps the lanes together */
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
             C v = x[k1];
-            if (k1 != 0 && n2 != 0) v = cmul(v, tw[(n2 * k1) % N]);
+            if (k1 != 0) v = cmul(v, tw[(n2 * k1) % N]);
             b[rpad(N2 * k1 + n2)] = v;
         }
     }
@@ -145,6 +147,35 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     const long comp = (long)N * N * A.kpad;
     C *out = (C *)A.shat;
     const T h = (T)0.5;
+    if constexpr (R == 1 && BLK == 2 * H) {
+        /* one k-row per workgroup (N = 256, 512): thread t keeps slot
+         * m = t mod H and takes sequences s0, s0 + 2, ... (s0 = t / H), i.e.
+         * pairs s0, s0 + 2, s0 + 4 of plane 0, then of plane 1.  The slot's
+         * LDS offsets and row address are computed once. */
+        const int m = threadIdx.x % H, s0 = threadIdx.x / H;
+        const bool slot0 = m == 0;   /* slot 0 packs S(0) + i S(N/2) */
+        const C *o = zo + s0 * SS;   /* sequence s0 + 2 q of plane pl at o + (6 pl + 2 q) SS */
+        const int iz = rpad(m), iw = rpad(slot0 ? N / 2 : N - m);
+        C *dst = out + (long)(2 * s0) * comp + ((long)i * N + j0) * A.kpad + m;
+        const long plane_stride = (long)N * A.kpad;
+        /* planes one at a time: unrolling both keeps too many values live */
+#pragma unroll 1
+        for (int pl = 0; pl < npl; pl++, o += 6 * SS, dst += plane_stride) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {   /* pair s0 + 2 q of plane pl */
+                const C Z = o[2 * q * SS + iz], W = o[2 * q * SS + iw];
+                C s0v, s1v;
+                s0v.x = slot0 ? Z.x : h * (Z.x + W.x);
+                s0v.y = slot0 ? W.x : h * (Z.y - W.y);
+                s1v.x = slot0 ? Z.y : h * (Z.y + W.y);
+                s1v.y = slot0 ? W.y : -h * (Z.x - W.x);
+                C *d = dst + (long)(4 * q) * comp;
+                d[0] = s0v;
+                d[comp] = s1v;
+            }
+        }
+        return;
+    }
     for (int task = threadIdx.x; task < nseq * H; task += BLK) {
         const int sq = task / H, m = task - sq * H;
         const int pl = sq / SPP, ps = sq - pl * SPP, pr = ps / R, r = ps - pr * R;
