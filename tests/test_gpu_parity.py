@@ -338,3 +338,50 @@ def test_rows_transformed_in_sweep_match_library_rows(precision, N, monkeypatch)
     hl, ll = lib.download_gw()
     tol = 1e-5 if fp32 else 1e-12
     assert rel_err(lf, ll) < tol and rel_err(hf, hl) < tol
+
+
+@pytest.mark.parametrize("ppb", [5, 6, 7])
+def test_plane_counts_per_workgroup_agree(ppb):
+    """Sweep workgroups marching 5, 6 or 7 planes give the fields and tensor
+    modes of the default tiling (1 plane at 32^3): FP32 sweep B's march
+    unrolled by four runs with a 1-, 2- or 3-plane tail, and sweep A's row
+    transforms end on a single plane.  The energy partials are grouped
+    differently, so a(t) may differ in its last bits: agreement to FP32
+    rounding."""
+    cfg = configs.lambda_phi4_config(N=32, calc_gws=1)
+    o = seeded_oracle(cfg, amp=2e-3)
+    ref = lattice_from_oracle(o, cfg, GABE_FP32)
+    lat = lattice_from_oracle(o, cfg, GABE_FP32, planes_per_block=ppb)
+    assert lat.query().planes_per_block == ppb
+    ref.step(8)
+    lat.step(8)
+    f0, d0 = ref.download_state()
+    f1, d1 = lat.download_state()
+    assert fluct_err(f1, f0) < 1e-6 and fluct_err(d1, d0) < 1e-6
+    h0, l0 = ref.download_gw()
+    h1, l1 = lat.download_gw()
+    assert rel_err(l1, l0) < 1e-5 and rel_err(h1, h0) < 1e-5
+
+
+def test_odd_plane_count_at_256():
+    """256^3 FP32 with 7 planes per sweep workgroup: the last row-transform call
+    of each workgroup holds one plane (the one-row split path with a single
+    plane), checked against the default 32-plane tiling on the reference's own
+    initial conditions generated on the device."""
+    from gabe_amd import host
+    cfg = configs.lambda_phi4_config(N=256, calc_gws=1)
+    ref = host.initialized_lattice(cfg, precision=GABE_FP32)
+    lat = host.initialized_lattice(cfg, precision=GABE_FP32, planes_per_block=7)
+    assert lat.query().planes_per_block == 7
+    ref.step(3)
+    lat.step(3)
+    h0, l0 = ref.download_gw()
+    h1, l1 = lat.download_gw()
+    assert rel_err(l1, l0) < 1e-5 and rel_err(h1, h0) < 1e-5
+    f0, _ = ref.download_state()
+    f1, _ = lat.download_state()
+    for f in range(f0.shape[0]):
+        fl = f0[f] - f0[f].mean()
+        assert np.linalg.norm(f1[f] - f0[f]) <= 1e-6 * np.linalg.norm(fl)
+    ref.close()
+    lat.close()
