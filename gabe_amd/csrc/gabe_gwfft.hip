@@ -49,8 +49,9 @@ __device__ __forceinline__ int trow2(int a, int b) { return (N2 + 1) * a + b; }
  * being linear in the components): load the CPR N x CW tiles, transform along
  * i (two steps, twiddles from twl), return to natural order, and project the
  * owned modes on e+ and ex into acc[mode][pol]. */
-template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0>
-__device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CPR][N1 * N2 + N1][CW],
+template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0, typename AfterDft>
+__device__ __forceinline__ void stage_rounds(AfterDft after_dft,
+                                             typename cplx_t<T>::type (&tile)[CPR][N1 * N2 + N1][CW],
                                              const typename cplx_t<T>::type *twl,
                                              const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
                                              const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
@@ -129,6 +130,9 @@ __device__ __forceinline__ void stage_rounds(typename cplx_t<T>::type (&tile)[CP
                 for (int k2 = 0; k2 < N2; k2++) tile[r][trow<N2>(k1 + N1 * k2)][col] = y[t][k2];
             }
         }
+        /* the transform's registers are free here: work that only needs to
+         * be in flight (the mode loads of the last round) starts now */
+        after_dft();
         __syncthreads();
 
         /* (e : S)/2 of the owned modes, off-diagonal components counted twice */
@@ -231,10 +235,40 @@ void k_gw_ifft(GwFftArgs<T> A)
 #pragma unroll
         for (int p = 0; p < 2; p++) { acc0[u][p].x = acc0[u][p].y = (T)0; acc1[u][p].x = acc1[u][p].y = (T)0; }
     }
-    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0>(tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
-    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3>(tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
-    stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
-    if constexpr (CPR == 3) stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3>(tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+    /* h and l of the owned modes, loaded during the last round (their latency
+     * hidden behind its projection) */
+    C *hh = reinterpret_cast<C *>(A.h);
+    C *ll = reinterpret_cast<C *>(A.l);
+    C h0v[MPT][2], l0v[MPT][2];
+    auto mode_index = [&](int u) -> long {
+        const int e = threadIdx.x + u * NT, i = e / CW, col = e % CW;
+        const int jj = j0 + col * jstep, pz = pz0 + col * pzstep;
+        /* tiled mode order [j][kz/4][i][kz%4] */
+        return (((long)jj * A.nt4 + (pz >> 2)) * N + i) * 4 + (pz & 3);
+    };
+    auto load_modes = [&]() {
+#pragma unroll
+        for (int u = 0; u < MPT; u++) {
+            const int e = threadIdx.x + u * NT, col = e % CW;
+            if (e >= N * CW || col >= width) continue;
+            const long m = mode_index(u);
+#pragma unroll
+            for (int pol = 0; pol < 2; pol++) {
+                h0v[u][pol] = hh[2 * m + pol];
+                l0v[u][pol] = ll[2 * m + pol];
+            }
+        }
+    };
+    auto nothing = [] {};
+    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
+    if constexpr (CPR == 3)
+        stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
+    if constexpr (CPR == 3) {
+        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+    } else {
+        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+    }
 
     /* both midpoint-RK2 stages of the owned modes (derivs, g2evolutionpert.cpp:486-501) */
     const StepScalars *ss = A.ss;
@@ -243,8 +277,6 @@ void k_gw_ifft(GwFftArgs<T> A)
     const T om0 = (T)(16. * M_PI / (ss->a0 * ss->a0) * A.dx3), om1 = (T)(16. * M_PI / (ss->a1 * ss->a1) * A.dx3);
     const double ia0 = 1. / (ss->a0 * ss->a0), ia1 = 1. / (ss->a1 * ss->a1);
     const T hdt = (T)(0.5 * A.dt), dt = (T)A.dt;
-    C *hh = reinterpret_cast<C *>(A.h);
-    C *ll = reinterpret_cast<C *>(A.l);
 #pragma unroll
     for (int u = 0; u < MPT; u++) {
         const int e = threadIdx.x + u * NT, i = e / CW, col = e % CW;
@@ -253,11 +285,10 @@ void k_gw_ifft(GwFftArgs<T> A)
         const int px = i <= N / 2 ? i : i - N, pz = pz0 + col * pzstep;
         const double p2 = (double)(px * px + py * py + pz * pz);
         const T kk0 = (T)(k2u * p2 * ia0), kk1 = (T)(k2u * p2 * ia1);
-        /* tiled mode order [j][kz/4][i][kz%4] */
-        const long m = (((long)jj * A.nt4 + (pz >> 2)) * N + i) * 4 + (pz & 3);
+        const long m = mode_index(u);
 #pragma unroll
         for (int pol = 0; pol < 2; pol++) {
-            const C h0 = hh[2 * m + pol], l0 = ll[2 * m + pol];
+            const C h0 = h0v[u][pol], l0 = l0v[u][pol];
             const C T0 = acc0[u][pol], T1 = acc1[u][pol];
             C h1, l1, hn, ln;
             h1.x = h0.x + hdt * l0.x;
