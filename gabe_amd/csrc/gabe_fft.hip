@@ -40,9 +40,10 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2;
-    static_assert(sizeof(C) * N * (CW + 2) <= 160 * 1024, "tile exceeds the LDS of one CU");
+    static_assert(sizeof(C) * N * (CW + 3) <= 160 * 1024, "tile exceeds the LDS of one CU");
     __shared__ C tile[N][CW + 1];
     __shared__ C packed[N];   /* transformed column 0 when it holds a packed pair */
+    __shared__ C twl[N];      /* the twiddle table, read by the first step */
     /* XCD-aware order: neighbouring column chunks on one XCD (one L2) */
     int b = blockIdx.x;
     if (gridDim.x % 8 == 0) b = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
@@ -51,22 +52,42 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
     const bool split = nyq != nullptr && sidx == 0 && q == 0;   /* uniform over the workgroup */
     C *base = data + (long)blockIdx.y * outer_stride + (long)sidx * sub_stride + (long)q * CW;
 
-    for (int task = threadIdx.x; task < N2 * CW; task += kColThreads) {
+    /* first step: task (n2, cc) loads x[n1] = row N2 n1 + n2, transforms it
+     * and multiplies by the twiddles; n2 = 0 multiplies by tw[0] = (1, -0),
+     * the value unchanged up to the sign of a zero, the lanes kept together */
+    auto first_step = [&](int task, C (&x)[N1]) {
         const int cc = task % CW, n2 = task / CW;
-        C x[N1];
+        dft_reg<N1, T>(x);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+            C v = x[k1];
+            if (k1 != 0) v = cmul(v, twl[(n2 * k1) % N]);
+            tile[N2 * k1 + n2][cc] = v;
+        }
+    };
+    auto load_task = [&](int task, C (&x)[N1]) {
+        const int cc = task % CW, n2 = task / CW;
 #pragma unroll
         for (int n1 = 0; n1 < N1; n1++) {
             if (cc < width) x[n1] = base[(long)(N2 * n1 + n2) * axis_stride + cc];
             else x[n1].x = x[n1].y = (T)0;
         }
-        dft_reg<N1, T>(x);
-        /* n2 = 0 multiplies by tw[0] = (1, -0): the value unchanged (up to the
-         * sign of a zero), the lanes kept together */
-#pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) {
-            C v = x[k1];
-            if (k1 != 0) v = cmul(v, tw[(n2 * k1) % N]);
-            tile[N2 * k1 + n2][cc] = v;
+    };
+    if constexpr (N2 * CW == kColThreads) {
+        /* one task per thread: its loads go out before the twiddle table is
+         * filled, so the barrier that publishes the table leaves them in flight */
+        C x[N1];
+        load_task(threadIdx.x, x);
+        for (int e = threadIdx.x; e < N; e += kColThreads) twl[e] = tw[e];
+        __syncthreads();
+        first_step(threadIdx.x, x);
+    } else {
+        for (int e = threadIdx.x; e < N; e += kColThreads) twl[e] = tw[e];
+        __syncthreads();
+        for (int task = threadIdx.x; task < N2 * CW; task += kColThreads) {
+            C x[N1];
+            load_task(task, x);
+            first_step(task, x);
         }
     }
     __syncthreads();
@@ -116,10 +137,10 @@ template <typename T, int N1, int N2>
 void launch_n(const ColFftView &v, int cw, const void *tw, hipStream_t s)
 {
     constexpr size_t csz = 2 * sizeof(T);
-    if constexpr (csz * N1 * N2 * 34 <= 160 * 1024) {
+    if constexpr (csz * N1 * N2 * 35 <= 160 * 1024) {
         if (cw == 32) { launch_cw<T, N1, N2, 32>(v, tw, s); return; }
     }
-    if constexpr (csz * N1 * N2 * 18 <= 160 * 1024) launch_cw<T, N1, N2, 16>(v, tw, s);
+    if constexpr (csz * N1 * N2 * 19 <= 160 * 1024) launch_cw<T, N1, N2, 16>(v, tw, s);
     else launch_cw<T, N1, N2, 8>(v, tw, s);
 }
 
