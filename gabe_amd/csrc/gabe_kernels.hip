@@ -114,7 +114,7 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     constexpr int R = BLK / N, SPP = 6 * R, H = N / 2;   /* sequences per plane, slots per row */
     constexpr int SS = rseq_stride(N);
     const int nseq = SPP * npl;
-    const C *tw = (const C *)A.tw;
+    const C *tw = zo;   /* the twiddle table, copied into zo before the call (step 2 overwrites it) */
     for (int task = threadIdx.x; task < nseq * N2; task += BLK) {
         const int sq = task / N2, n2 = task - sq * N2;
         C *b = zl + sq * SS;
@@ -525,6 +525,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             constexpr int PPC = row_planes<T>(ROWN);
             if ((i - i0) % PPC == PPC - 1 || i + 1 == i1) {
                 const int npl = (i - i0) % PPC + 1;
+                /* zo is free until the transform's second step: it holds the
+                 * twiddle table for the first */
+                using C = typename gabe_fft::cplx_t<T>::type;
+                for (int e = threadIdx.x; e < ROWN; e += BLK) zo[e] = reinterpret_cast<const C *>(A.tw)[e];
                 __syncthreads();
                 row_r2c<T, ROWN>(zl, zo, A, i + 1 - npl, bx * (BLK / ROWN), npl);
                 __syncthreads();
