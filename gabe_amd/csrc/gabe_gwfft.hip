@@ -328,18 +328,19 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
     case 128: launch_cfg<float, 16, 8, 8, 6, 1024>(a, s); break;
     case 256: {
         /* tile variants (kz columns per workgroup, components per round,
-         * threads); the default, 4 columns and all 6 components of a stage
-         * per round on 512 threads (two workgroups per CU), measured fastest
-         * once the tile rows were made bank-conflict free */
+         * threads); the default, 4 columns and 3 components per round on 256
+         * threads (3 workgroups per CU), measured fastest once the mode loads
+         * moved into the last round; 4 columns x 6 components on 512 threads
+         * was faster before that */
         const char *v = std::getenv("GABE_GWFFT_TILE");
         const int t = v ? std::atoi(v) : 0;
-        if (t == 1) launch_cfg<float, 16, 16, 4, 3, 256>(a, s);
+        if (t == 1) launch_cfg<float, 16, 16, 4, 6, 512>(a, s);
         else if (t == 2) launch_cfg<float, 16, 16, 4, 6, 256>(a, s);
         else if (t == 3) launch_cfg<float, 16, 16, 8, 6, 512>(a, s);
         else if (t == 4) launch_cfg<float, 16, 16, 8, 3, 512>(a, s);
         else if (t == 5) launch_cfg<float, 16, 16, 4, 3, 512>(a, s);
         else if (t == 6) launch_cfg<float, 16, 16, 8, 6, 1024>(a, s);
-        else launch_cfg<float, 16, 16, 4, 6, 512>(a, s);
+        else launch_cfg<float, 16, 16, 4, 3, 256>(a, s);
         break;
     }
     case 512: {
