@@ -25,10 +25,11 @@ namespace {
 
 using namespace gabe_fft;
 
-/* largest divisor of n not above 8: tile loads in flight per batch */
+/* largest divisor of n not above 12: tile loads in flight per batch (all 12
+ * of the FP32 256^3 tile at once) */
 constexpr int load_batch(int n)
 {
-    for (int b = 8; b > 1; b--)
+    for (int b = 12; b > 1; b--)
         if (n % b == 0) return b;
     return 1;
 }
@@ -44,19 +45,28 @@ __device__ __forceinline__ int trow(int i) { return i + i / N2; }
 template <int N2>
 __device__ __forceinline__ int trow2(int a, int b) { return (N2 + 1) * a + b; }
 
+/* tile elements each thread loads per round */
+constexpr int gw_lpt(int N, int CW, int CPR, int NT) { return (CPR * N * CW + NT - 1) / NT; }
+
 /* One round of an RK2 stage: components 6 ST + C0 .. 6 ST + C0 + CPR - 1 of
  * the source (CPR = 6: the whole stage; CPR = 3: half of it, the projection
- * being linear in the components): load the CPR N x CW tiles, transform along
- * i (two steps, twiddles from twl), return to natural order, and project the
- * owned modes on e+ and ex into acc[mode][pol]. */
-template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0, typename AfterDft>
+ * being linear in the components): fill the CPR N x CW tiles (loaded now, or
+ * with PF_IN from pf, loaded by the round before), transform along i (two
+ * steps, twiddles from twl), return to natural order, and project the owned
+ * modes on e+ and ex into acc[mode][pol].  With PF_OUT the next round's tiles
+ * are loaded into pf once the transform is done, so their latency passes
+ * behind the projection and the barriers around it. */
+
+template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0, bool PF_IN, bool PF_OUT,
+          typename AfterDft>
 __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
                                              typename cplx_t<T>::type (&tile)[CPR][N1 * N2 + N1][CW],
                                              const typename cplx_t<T>::type *twl,
                                              const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
                                              const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
                                              const T (&bv)[(N1 * N2 * CW + NT - 1) / NT][3],
-                                             typename cplx_t<T>::type (&acc)[(N1 * N2 * CW + NT - 1) / NT][2])
+                                             typename cplx_t<T>::type (&acc)[(N1 * N2 * CW + NT - 1) / NT][2],
+                                             typename cplx_t<T>::type (&pf)[gw_lpt(N1 * N2, CW, CPR, NT)])
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2;
@@ -69,6 +79,14 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
     static_assert(LPT % LB == 0, "loads must split into whole batches");
     {
         constexpr int c0 = C0;
+        if constexpr (PF_IN) {
+#pragma unroll
+            for (int t = 0; t < LPT; t++) {
+                const int e = threadIdx.x + t * NT;
+                const int r = e / (N * CW), rem = e % (N * CW);
+                if (e < CPR * N * CW) tile[r][trow<N2>(rem / CW)][rem % CW] = pf[t];
+            }
+        } else
 #pragma unroll
         for (int t0 = 0; t0 < LPT; t0 += LB) {
             C v[LB];
@@ -133,6 +151,16 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
         /* the transform's registers are free here: work that only needs to
          * be in flight (the mode loads of the last round) starts now */
         after_dft();
+        if constexpr (PF_OUT) {
+#pragma unroll
+            for (int t = 0; t < LPT; t++) {
+                const int e = threadIdx.x + t * NT;
+                const int r = e / (N * CW), rem = e % (N * CW), i = rem / CW, col = rem % CW;
+                if (e < CPR * N * CW && col < width)
+                    pf[t] = Sh[(long)(6 * ST + c0 + CPR + r) * SM + (long)i * rstride + col];
+                else pf[t].x = pf[t].y = (T)0;
+            }
+        }
         __syncthreads();
 
         /* (e : S)/2 of the owned modes, off-diagonal components counted twice */
@@ -260,14 +288,17 @@ void k_gw_ifft(GwFftArgs<T> A)
         }
     };
     auto nothing = [] {};
-    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
-    if constexpr (CPR == 3)
-        stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0);
+    /* the next round's tile, carried in registers from one round to the next */
+    C pf[gw_lpt(N, CW, CPR, NT)];
+    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0, false, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0, pf);
     if constexpr (CPR == 3) {
-        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
-        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+        stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3, true, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0, pf);
+        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0, true, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc1, pf);
+        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3, true, false>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1,
+                                                                 pf);
     } else {
-        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1);
+        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0, true, false>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1,
+                                                                 pf);
     }
 
     /* both midpoint-RK2 stages of the owned modes (derivs, g2evolutionpert.cpp:486-501) */
