@@ -54,8 +54,9 @@ constexpr int gw_lpt(int N, int CW, int CPR, int NT) { return (CPR * N * CW + NT
  * with PF_IN from pf, loaded by the round before), transform along i (two
  * steps, twiddles from twl), return to natural order, and project the owned
  * modes on e+ and ex into acc[mode][pol].  With PF_OUT the next round's tiles
- * are loaded into pf once the transform is done, so their latency passes
- * behind the projection and the barriers around it. */
+ * are loaded into pf after the first DFT step, so their latency passes behind
+ * the second step, the projection and the barriers between them (loaded
+ * before the first step they would not fit the register budget). */
 
 template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0, bool PF_IN, bool PF_OUT,
           typename AfterDft>
@@ -79,6 +80,16 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
     static_assert(LPT % LB == 0, "loads must split into whole batches");
     {
         constexpr int c0 = C0;
+        auto prefetch = [&] {
+#pragma unroll
+            for (int t = 0; t < LPT; t++) {
+                const int e = threadIdx.x + t * NT;
+                const int r = e / (N * CW), rem = e % (N * CW), i = rem / CW, col = rem % CW;
+                if (e < CPR * N * CW && col < width)
+                    pf[t] = Sh[(long)(6 * ST + c0 + CPR + r) * SM + (long)i * rstride + col];
+                else pf[t].x = pf[t].y = (T)0;
+            }
+        };
         if constexpr (PF_IN) {
 #pragma unroll
             for (int t = 0; t < LPT; t++) {
@@ -126,6 +137,7 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
             }
         }
         __syncthreads();
+        if constexpr (PF_OUT) prefetch();
 
         C y[S2T][N2];
 #pragma unroll
@@ -151,16 +163,6 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
         /* the transform's registers are free here: work that only needs to
          * be in flight (the mode loads of the last round) starts now */
         after_dft();
-        if constexpr (PF_OUT) {
-#pragma unroll
-            for (int t = 0; t < LPT; t++) {
-                const int e = threadIdx.x + t * NT;
-                const int r = e / (N * CW), rem = e % (N * CW), i = rem / CW, col = rem % CW;
-                if (e < CPR * N * CW && col < width)
-                    pf[t] = Sh[(long)(6 * ST + c0 + CPR + r) * SM + (long)i * rstride + col];
-                else pf[t].x = pf[t].y = (T)0;
-            }
-        }
         __syncthreads();
 
         /* (e : S)/2 of the owned modes, off-diagonal components counted twice */
@@ -288,7 +290,7 @@ void k_gw_ifft(GwFftArgs<T> A)
         }
     };
     auto nothing = [] {};
-    /* the next round's tile, carried in registers from one round to the next */
+    /* the next round's tiles, carried in registers from one round to the next */
     C pf[gw_lpt(N, CW, CPR, NT)];
     stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0, false, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0, pf);
     if constexpr (CPR == 3) {
