@@ -51,11 +51,9 @@ __device__ __forceinline__ double wave_sum(double v)
  * A workgroup of WIDE x GABE_BLOCK threads stands for WIDE consecutive
  * GABE_BLOCK workgroups: it writes its sum into the first of their slots and
  * zeros into the others, so the finalize reads the same nblk slots. */
-template <int NS, int WIDE = 1>
-__device__ void block_store_partials(double (&acc)[NS], double *partials, int nblk, int bid)
+template <int NS, int WIDE>
+__device__ void reduce_partials(double (&acc)[NS], double (*red)[NS], double *partials, int nblk, int bid)
 {
-    constexpr int NWV = WIDE * GABE_BLOCK / 64;
-    __shared__ double red[NWV][NS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -70,6 +68,15 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
         partials[(long)s * nblk + WIDE * bid] = tot;
         for (int q = 1; q < WIDE; q++) partials[(long)s * nblk + WIDE * bid + q] = 0.;
     }
+}
+
+/* the same with an LDS area of its own; workgroups with a free LDS buffer
+ * call reduce_partials on it instead */
+template <int NS, int WIDE = 1>
+__device__ void block_store_partials(double (&acc)[NS], double *partials, int nblk, int bid)
+{
+    __shared__ double red[WIDE * GABE_BLOCK / 64][NS];
+    reduce_partials<NS, WIDE>(acc, red, partials, nblk, bid);
 }
 
 /* ------------------------------------------------------------------ sweeps */
@@ -104,7 +111,8 @@ __host__ __device__ constexpr int rbuf_size(int N) { return row_planes<T>(N) * 6
 
 template <typename T, int N>
 __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, typename gabe_fft::cplx_t<T>::type *zo,
-                                       const SweepArgs<T> &A, int i, int j0, int npl)
+                                       const typename gabe_fft::cplx_t<T>::type *tw, const SweepArgs<T> &A, int i,
+                                       int j0, int npl)
 {
     using gabe_fft::cmul;
     using C = typename gabe_fft::cplx_t<T>::type;
@@ -113,8 +121,8 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     constexpr int BLK = row_block(N);
     constexpr int R = BLK / N, SPP = 6 * R, H = N / 2;   /* sequences per plane, slots per row */
     constexpr int SS = rseq_stride(N);
+    static_assert((N1 - 1) * (N2 - 1) < N, "twiddle exponents index one table of N");
     const int nseq = SPP * npl;
-    const C *tw = zo;   /* the twiddle table, copied into zo before the call (step 2 overwrites it) */
     for (int task = threadIdx.x; task < nseq * N2; task += BLK) {
         const int sq = task / N2, n2 = task - sq * N2;
         C *b = zl + sq * SS;
@@ -127,7 +135,14 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
             C v = x[k1];
-            if (k1 != 0) v = cmul(v, tw[(n2 * k1) % N]);
+            if (k1 != 0) {
+                /* W^e for e = n2 k1 < N, from the half table: W^(e + N/2) = -W^e */
+                const int e = n2 * k1;
+                const bool ng = e >= N / 2;
+                C w = tw[ng ? e - N / 2 : e];
+                if (ng) { w.x = -w.x; w.y = -w.y; }
+                v = cmul(v, w);
+            }
             b[rpad(N2 * k1 + n2)] = v;
         }
     }
@@ -334,7 +349,8 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
 template <typename T, int NF, int MODE, bool GW, int ROWN = 0>
 __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy,
                                            typename gabe_fft::cplx_t<T>::type *zl = nullptr,
-                                           typename gabe_fft::cplx_t<T>::type *zo = nullptr)
+                                           typename gabe_fft::cplx_t<T>::type *zo = nullptr,
+                                           typename gabe_fft::cplx_t<T>::type *zt = nullptr)
 {
     if constexpr (MODE == SW_B && sizeof(T) == 4) {
         sweep_b_ahead<T, NF>(A, hw, gx, gy);
@@ -401,6 +417,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
     double acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) acc[s] = 0.;
+    if constexpr (ROWN > 0) {
+        /* the first half of the row transform's twiddle table, read from LDS
+         * by every call; the barrier before the first call publishes it */
+        using C = typename gabe_fft::cplx_t<T>::type;
+        for (int e = threadIdx.x; e < ROWN / 2; e += BLK) zt[e] = reinterpret_cast<const C *>(A.tw)[e];
+    }
 
     T fm[NF], fc[NF], vm[NF], vc[NF];
     const int im0 = i0 == 0 ? N - 1 : i0 - 1;
@@ -525,13 +547,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             constexpr int PPC = row_planes<T>(ROWN);
             if ((i - i0) % PPC == PPC - 1 || i + 1 == i1) {
                 const int npl = (i - i0) % PPC + 1;
-                /* zo is free until the transform's second step: it holds the
-                 * twiddle table for the first */
-                using C = typename gabe_fft::cplx_t<T>::type;
-                for (int e = threadIdx.x; e < ROWN; e += BLK) zo[e] = reinterpret_cast<const C *>(A.tw)[e];
                 __syncthreads();
-                row_r2c<T, ROWN>(zl, zo, A, i + 1 - npl, bx * (BLK / ROWN), npl);
-                __syncthreads();
+                /* no barrier after the call: the next rows written to zl
+                 * follow its last reads of zl (before its final barrier), and
+                 * the next writes to zo come after the next call's first two
+                 * barriers */
+                row_r2c<T, ROWN>(zl, zo, zt, A, i + 1 - npl, bx * (BLK / ROWN), npl);
             }
         }
 #pragma unroll
@@ -540,7 +561,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             vm[f] = vc[f]; vc[f] = vp[f];
         }
     }
-    if (MODE != SW_B) block_store_partials<NS, BLK / GABE_BLOCK>(acc, A.partials, A.nblk, bid);
+    if constexpr (ROWN > 0) {
+        /* zl is free: the last transform's reads of it came before its final
+         * barrier; using it keeps three workgroups' LDS within one CU */
+        static_assert(sizeof(double) * (BLK / 64) * NS <= sizeof(*zl) * rbuf_size<T>(ROWN), "reduction exceeds zl");
+        reduce_partials<NS, BLK / GABE_BLOCK>(acc, reinterpret_cast<double (*)[NS]>(zl), A.partials, A.nblk, bid);
+    } else if (MODE != SW_B) {
+        block_store_partials<NS, BLK / GABE_BLOCK>(acc, A.partials, A.nblk, bid);
+    }
 }
 
 /* sweep A with the stress rows transformed in the workgroup (N 16..512) */
@@ -548,8 +576,11 @@ template <typename T, int NF, int ROWN>
 __global__ __launch_bounds__(row_block(ROWN)) void k_sweep_rowfft(SweepArgs<T> A)
 {
     using C = typename gabe_fft::cplx_t<T>::type;
-    __shared__ C zl[rbuf_size<T>(ROWN)], zo[rbuf_size<T>(ROWN)];
-    sweep_body<T, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo);
+    /* zl, zo and the half twiddle table: 53,248 bytes at 256 sites in FP32,
+     * so that three workgroups fit in the LDS of one CU (a whole table, 2 KB
+     * more, measured at two) */
+    __shared__ C zl[rbuf_size<T>(ROWN)], zo[rbuf_size<T>(ROWN)], zt[ROWN / 2];
+    sweep_body<T, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo, zt);
 }
 
 template <typename T, int NF, int MODE, bool GW>
