@@ -642,22 +642,25 @@ __device__ __forceinline__ void finalize_body(const FinArgs &F)
         for (int f = 0; f < NF; f++) { pbv[f] = sc->pbar[f]; vbv[f] = sc->vbar[f]; }
     }
 
-    /* each thread adds partials b, b + GABE_BLOCK, ... in that order; four of
-     * them are loaded at a time so that one wait covers 4 NS loads */
+    /* each thread adds partials b, b + GABE_BLOCK, ... in that order; two of
+     * them are loaded at a time so that one wait covers 2 NS loads.  Four at a
+     * time took 250 VGPRs per wave: beside the mode kernel (168 per wave, one
+     * wave per SIMD per workgroup) the finalize then waited for two of its
+     * workgroups to leave one CU; at 164 one is enough */
     double s[NS], c[NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) { s[q] = 0.; c[q] = 0.; }
     const int nb = F.nblk;
     int b = threadIdx.x;
-    for (; b + 3 * GABE_BLOCK < nb; b += 4 * GABE_BLOCK) {
-        double v[4][NS];
+    for (; b + GABE_BLOCK < nb; b += 2 * GABE_BLOCK) {
+        double v[2][NS];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < 2; u++)
 #pragma unroll
             for (int q = 0; q < NS; q++) v[u][q] = F.partials[(long)q * nb + b + u * GABE_BLOCK];
         __builtin_amdgcn_sched_barrier(0);   /* all loads issued before the first add */
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < 2; u++)
 #pragma unroll
             for (int q = 0; q < NS; q++) {
                 double t, e;
