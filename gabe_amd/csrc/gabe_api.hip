@@ -545,6 +545,16 @@ int download_fields(gabe_ctx *c, const void *src, double *dst, const double *mea
     return GABE_OK;
 }
 
+/* nsteps steps, then the side stream's work joined into the main stream, so
+ * every later synchronisation of the main stream covers both */
+int run_steps(gabe_ctx *c, int nsteps)
+{
+    for (int n = 0; n < nsteps; n++) CHK(one_step(c, nullptr));
+    if (c->pipe)
+        for (hipEvent_t e : c->ev_gw) HIPCHK(hipStreamWaitEvent(c->stream, e, 0));
+    return GABE_OK;
+}
+
 } // namespace gabe_detail
 
 /* ====================================================================== ABI */
@@ -895,12 +905,7 @@ int gabe_step(gabe_ctx *c, int nsteps)
 {
     if (!c) return fail(GABE_ERR_ARG, "NULL context");
     if (nsteps < 0) return fail(GABE_ERR_ARG, "nsteps < 0");
-    for (int n = 0; n < nsteps; n++) CHK(one_step(c, nullptr));
-    /* the tensor-mode work of the last steps joins the main stream, so every
-     * later synchronisation of it covers the side stream too */
-    if (c->pipe)
-        for (hipEvent_t e : c->ev_gw) HIPCHK(hipStreamWaitEvent(c->stream, e, 0));
-    return GABE_OK;
+    return run_steps(c, nsteps);
 }
 
 int gabe_sync(gabe_ctx *c)
@@ -1233,7 +1238,7 @@ int time_steps(gabe_ctx *c, int steps, float &ms)
     HIPCHK(hipEventCreate(&e1));
     int rc = GABE_OK;
     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GABE_ERR_HIP, "event record failed");
-    for (int n = 0; n < steps && rc == GABE_OK; n++) rc = one_step(c, nullptr);
+    if (rc == GABE_OK) rc = run_steps(c, steps);
     if (rc == GABE_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
         rc = fail(GABE_ERR_HIP, "event sync failed");
     if (rc == GABE_OK) (void)hipEventElapsedTime(&ms, e0, e1);
