@@ -131,30 +131,13 @@ def precision_report(f32, f64):
     return out
 
 
-def ensemble_members(sizes=(32, 64, 96, 128), per_size=16, seed=20240):
-    """64 independent lattices: 16 each of 32^3, 64^3, 96^3, 128^3 with L = 20,
-    dt = dx/20, distinct IC seeds and g^2/lambda log-uniform in [1, 200]; no
-    tensor modes (the batched path steps fields and expansion only)."""
-    import math
-    import random
-    from gabe_amd import configs
-    rng = random.Random(seed)
-    cfgs = []
-    for N in sizes:
-        for _ in range(per_size):
-            ratio = math.exp(rng.uniform(0.0, math.log(200.0)))
-            c = configs.lambda_phi4_config(N=N, calc_gws=0, g2_over_lambda=ratio)
-            c.randseed = 1 + len(cfgs) * 7919
-            cfgs.append(c)
-    return cfgs
-
-
 def run_ensemble(steps, warmup):
     """Aggregate site-updates/s of the batched ensemble against the same 64
     lattices run one after another, each to completion, on their own contexts."""
     from gabe_amd import host
     from gabe_amd.lattice import GABE_FP32, Ensemble, Lattice, autotune_sweep
-    cfgs = ensemble_members()
+    from gabe_amd import configs
+    cfgs = configs.ensemble_configs()
     sites = sum(c.N ** 3 for c in cfgs)
     tuned = {}
     for c in cfgs:
@@ -240,13 +223,17 @@ def run_closure(steps, warmup):
     return out
 
 
-def cpu_baseline(cfg, state, nsteps=2):
-    """The CPU FP64 oracle, OpenMP over the outer index, on this host's cores."""
+def _oracle_for(cfg, threads):
     from oracle import pyoracle as po
-    threads = min(16, os.cpu_count() or 1)
-    o = po.Oracle(cfg.N, nflds=cfg.nflds, L=cfg.L, dt=cfg.dt, calc_gws=cfg.calc_gws,
-                  expansion_type=cfg.expansion_type, m2_phi=cfg.m2_phi, lam=cfg.lam, g2=cfg.g2,
-                  m2_chi=cfg.m2_chi, rescale_B=cfg.rescale_B, nthreads=threads)
+    return po.Oracle(cfg.N, nflds=cfg.nflds, L=cfg.L, dt=cfg.dt, calc_gws=cfg.calc_gws,
+                     expansion_type=cfg.expansion_type, m2_phi=cfg.m2_phi, lam=cfg.lam, g2=cfg.g2,
+                     m2_chi=cfg.m2_chi, rescale_B=cfg.rescale_B, nthreads=threads)
+
+
+def _time_oracle(cfg, state, threads, nsteps):
+    """nsteps oracle steps from `state` (fields and velocities, FP64) with
+    `threads` OpenMP threads; returns site-updates/s and seconds."""
+    o = _oracle_for(cfg, threads)
     f, d = state
     o.field[0] = f
     o.dfield[0] = d
@@ -255,14 +242,53 @@ def cpu_baseline(cfg, state, nsteps=2):
     o.set_scalar("adot", 0, o.adf(0))
     t0 = time.perf_counter()
     o.step(nsteps)
-    t1 = time.perf_counter()
+    dt = time.perf_counter() - t0
+    del o
+    return cfg.N ** 3 * nsteps / dt, dt
+
+
+def cpu_baseline(cfg, state, nsteps=2):
+    """The CPU FP64 oracle (the reference's loops restated in C, OpenMP over
+    the outer index) on this host's cores: the benchmark lattice (2 steps) on
+    the host's CPU share and on 4 threads (the reference's default thread
+    count, tot_num_thrds in g2parameters.h:54), and the plain 64^3 workload
+    without tensor modes over its full 200 steps on both thread counts, with
+    the same 200 steps on the GPU in FP64 beside them."""
+    from gabe_amd import configs, host
+    from gabe_amd.lattice import GABE_FP64
+    nproc = os.cpu_count() or 1
+    share = min(16, nproc)   # one GPU's share of the box's host cores
+    main_rate, main_s = _time_oracle(cfg, state, share, nsteps)
+    rate4, s4 = _time_oracle(cfg, state, 4, nsteps)
+    c1 = configs.lambda_phi4_config(N=64, calc_gws=0)
+    lat = host.initialized_lattice(c1, precision=GABE_FP64)
+    f1, d1 = lat.download_state()
+    lat.sync()
+    t0 = time.perf_counter()
+    lat.step(200)
+    lat.sync()
+    gpu_s = time.perf_counter() - t0
+    lat.close()
+    c1_share, c1_share_s = _time_oracle(c1, (f1, d1), share, 200)
+    c1_four, c1_four_s = _time_oracle(c1, (f1, d1), 4, 200)
+    del f1, d1
     return {
-        "value": cfg.N ** 3 * nsteps / (t1 - t0),
+        "value": main_rate,
         "unit": "site-updates/s",
-        "cores": threads,
+        "cores": share,
         "kind": "port",
-        "sample": f"{nsteps} RK2 steps of the same {cfg.N}^3 2-field+GW lattice in FP64 "
-                  f"(oracle restatement, OpenMP over i, {threads} threads), {t1 - t0:.1f} s",
+        "host_cpus": nproc,
+        "sample": f"{nsteps} RK2 steps of the same {cfg.N}^3 2-field+GW lattice in FP64 (oracle restatement, "
+                  f"OpenMP over i, {share} threads = one GPU's share of the {nproc} host CPUs), {main_s:.1f} s",
+        "threads_4": {"value": rate4, "cores": 4, "seconds": s4,
+                      "sample": f"the same {nsteps} steps on 4 threads (tot_num_thrds, g2parameters.h:54)"},
+        "plain_64_200_steps": {
+            "workload": "64^3, 2 fields, no GW, FP64, 200 RK2 steps from the seeded vacuum state",
+            "cpu": {f"threads_{share}": {"value": c1_share, "seconds": c1_share_s},
+                    "threads_4": {"value": c1_four, "seconds": c1_four_s}},
+            "gpu_fp64": {"value": 64 ** 3 * 200 / gpu_s, "seconds": gpu_s},
+            "unit": "site-updates/s",
+        },
     }
 
 

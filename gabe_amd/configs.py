@@ -63,3 +63,20 @@ def lambda_phi4_config(N=256, L=20.0, calc_gws=1, g2_over_lambda=100.0, phi0=0.3
     dx = L / N
     return ModelConfig(N=N, L=L, dt=dx / 20.0, nflds=2, m2_phi=0.0, lam=lam_pr, g2=g2_pr,
                        m2_chi=0.0, f0=(phi0, 0.0), df0=(dphi0, 0.0), rescale_B=B, calc_gws=calc_gws)
+
+
+def ensemble_configs(sizes=(32, 64, 96, 128), per_size=16, seed=20240):
+    """The batched-ensemble workload: 16 each of 32^3, 64^3, 96^3 and 128^3
+    lattices with L = 20, dt = dx/20, distinct IC seeds and g^2/lambda drawn
+    log-uniformly from [1, 200]; no tensor modes (the batched path steps the
+    fields and the expansion only)."""
+    import random
+    rng = random.Random(seed)
+    cfgs = []
+    for N in sizes:
+        for _ in range(per_size):
+            ratio = math.exp(rng.uniform(0.0, math.log(200.0)))
+            c = lambda_phi4_config(N=N, calc_gws=0, g2_over_lambda=ratio)
+            c.randseed = 1 + len(cfgs) * 7919
+            cfgs.append(c)
+    return cfgs

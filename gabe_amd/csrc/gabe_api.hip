@@ -673,25 +673,30 @@ int gabe_means_vars(gabe_ctx *c, double *out)
     return GABE_OK;
 }
 
-/* Checkpoint blob: a header, then the device scalars, the history rows in
- * use (oldest first), the current slot's field and velocity arrays and, with
- * tensor modes, h and l, all in the device's own storage precision, so a
- * restored context continues bit for bit. */
+/* Checkpoint blob: a header, then the device scalars, the history rows of the
+ * current trajectory (oldest first), the current slot's field and velocity
+ * arrays, with tensor modes h and l (in the storage order the header names),
+ * and the attached learned closure's state (gate, fallback step, largest
+ * residual, threshold, scales, weights) if one is attached, all in the
+ * device's own storage precision, so a restored context continues bit for
+ * bit.  Loading a blob without a closure section detaches any closure. */
 struct CkptHeader {
     char magic[8];
     unsigned version;
     int N, nflds, precision, calc_gws, expansion_type;
     double L, dt, m2_phi, lambda, g2, m2_chi;
     int gw_spectrum_quirk, hist_rows;
-    unsigned long long state_bytes, gw_bytes, total;
+    int mode_tiled, mode_nt4;     /* storage order of the tensor-mode pairs (ModeLayout) */
+    long long mode_kp;
+    unsigned long long state_bytes, gw_bytes, closure_bytes, total;
 };
 static const char kCkptMagic[8] = {'G', 'A', 'B', 'E', 'S', 'T', 'A', 'T'};
 
-static CkptHeader ckpt_header(const gabe_ctx *c, int hist_rows)
+static CkptHeader ckpt_header(const gabe_ctx *c, int hist_rows, unsigned long long closure_bytes)
 {
     CkptHeader h{};
     std::memcpy(h.magic, kCkptMagic, 8);
-    h.version = 1;
+    h.version = 2;
     h.N = c->N;
     h.nflds = c->nf;
     h.precision = c->p.precision;
@@ -705,11 +710,21 @@ static CkptHeader ckpt_header(const gabe_ctx *c, int hist_rows)
     h.m2_chi = c->p.m2_chi;
     h.gw_spectrum_quirk = c->p.gw_spectrum_quirk;
     h.hist_rows = hist_rows;
+    h.mode_tiled = c->alloc_gw ? c->lay.tiled : 0;
+    h.mode_nt4 = c->alloc_gw ? c->lay.nt4 : 0;
+    h.mode_kp = c->alloc_gw ? c->lay.KP : 0;
     h.state_bytes = (unsigned long long)c->nf * c->N3 * c->esz;
     h.gw_bytes = c->alloc_gw ? 2ULL * c->esz * 2 * c->mslots : 0;
+    h.closure_bytes = closure_bytes;
     h.total = sizeof(CkptHeader) + sizeof(DevScalars) + sizeof(double) * GABE_HIST_COLS * (unsigned long long)hist_rows +
-              2 * h.state_bytes + 2 * h.gw_bytes;
+              2 * h.state_bytes + 2 * h.gw_bytes + closure_bytes;
     return h;
+}
+
+/* history rows of the current trajectory held in the ring */
+static int hist_rows_of(const gabe_ctx *c, const DevScalars &d)
+{
+    return (int)std::min<long long>(std::max<long long>(0, d.nstep - d.hist0), c->hist_cap);
 }
 
 int gabe_checkpoint_size(gabe_ctx *c, size_t *bytes)
@@ -717,7 +732,7 @@ int gabe_checkpoint_size(gabe_ctx *c, size_t *bytes)
     if (!c || !bytes) return fail(GABE_ERR_ARG, "NULL argument");
     DevScalars d;
     CHK(read_scalars(c, d));
-    *bytes = ckpt_header(c, (int)std::min<long long>(d.nstep, c->hist_cap)).total;
+    *bytes = ckpt_header(c, hist_rows_of(c, d), closure_ckpt_bytes(c)).total;
     return GABE_OK;
 }
 
@@ -726,8 +741,8 @@ int gabe_checkpoint_save(gabe_ctx *c, void *buf, size_t bytes)
     if (!c || !buf) return fail(GABE_ERR_ARG, "NULL argument");
     DevScalars d;
     CHK(read_scalars(c, d));
-    const int rows = (int)std::min<long long>(d.nstep, c->hist_cap);
-    const CkptHeader h = ckpt_header(c, rows);
+    const int rows = hist_rows_of(c, d);
+    const CkptHeader h = ckpt_header(c, rows, closure_ckpt_bytes(c));
     if (bytes < h.total) return fail(GABE_ERR_ARG, "checkpoint buffer holds %zu bytes, %llu needed", bytes, h.total);
     char *p = (char *)buf;
     std::memcpy(p, &h, sizeof h);
@@ -752,6 +767,7 @@ int gabe_checkpoint_save(gabe_ctx *c, void *buf, size_t bytes)
             HIPCHK(hipMemcpy(p, src, h.gw_bytes, hipMemcpyDeviceToHost));
             p += h.gw_bytes;
         }
+    CHK(closure_ckpt_save(c, p));
     return GABE_OK;
 }
 
@@ -761,21 +777,27 @@ int gabe_checkpoint_load(gabe_ctx *c, const void *buf, size_t bytes)
     if (bytes < sizeof(CkptHeader)) return fail(GABE_ERR_ARG, "checkpoint shorter than its header");
     CkptHeader h;
     std::memcpy(&h, buf, sizeof h);
-    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) return fail(GABE_ERR_ARG, "not a gabe checkpoint");
-    const CkptHeader mine = ckpt_header(c, h.hist_rows);
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0) return fail(GABE_ERR_ARG, "not a gabe checkpoint");
+    if (h.version != 2) return fail(GABE_ERR_ARG, "checkpoint format version %u; this library reads version 2", h.version);
+    const CkptHeader mine = ckpt_header(c, h.hist_rows, h.closure_bytes);
     if (h.N != mine.N || h.nflds != mine.nflds || h.precision != mine.precision || h.calc_gws != mine.calc_gws ||
         h.expansion_type != mine.expansion_type || h.L != mine.L || h.dt != mine.dt || h.m2_phi != mine.m2_phi ||
         h.lambda != mine.lambda || h.g2 != mine.g2 || h.m2_chi != mine.m2_chi ||
         h.gw_spectrum_quirk != mine.gw_spectrum_quirk)
         return fail(GABE_ERR_ARG, "checkpoint was written by a context with other parameters (N=%d nflds=%d "
                                   "precision=%d calc_gws=%d)", h.N, h.nflds, h.precision, h.calc_gws);
+    if (h.mode_tiled != mine.mode_tiled || h.mode_nt4 != mine.mode_nt4 || h.mode_kp != mine.mode_kp)
+        return fail(GABE_ERR_ARG, "checkpoint stores its tensor modes in another order (%s, %lld slots per row) than "
+                                  "this lattice (%s, %lld): it was written with other GW transform settings",
+                    h.mode_tiled ? "tiled" : "row-major", h.mode_kp, mine.mode_tiled ? "tiled" : "row-major",
+                    mine.mode_kp);
     if (h.hist_rows < 0 || h.hist_rows > c->hist_cap || h.total != mine.total || bytes < h.total)
         return fail(GABE_ERR_ARG, "checkpoint size does not match its header");
     const char *p = (const char *)buf + sizeof h;
     DevScalars d;
     std::memcpy(&d, p, sizeof d);
     p += sizeof d;
-    if (d.nstep < h.hist_rows) return fail(GABE_ERR_ARG, "checkpoint history longer than its step count");
+    if (h.hist_rows != hist_rows_of(c, d)) return fail(GABE_ERR_ARG, "checkpoint history does not match its step count");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (h.hist_rows) {
         std::vector<double> ring((size_t)c->hist_cap * GABE_HIST_COLS, 0.);
@@ -795,6 +817,7 @@ int gabe_checkpoint_load(gabe_ctx *c, const void *buf, size_t bytes)
             HIPCHK(hipMemcpy(dst, p, h.gw_bytes, hipMemcpyHostToDevice));
             p += h.gw_bytes;
         }
+    CHK(closure_ckpt_load(c, p, h.closure_bytes));
     CHK(write_scalars(c, d));
     return GABE_OK;
 }
@@ -863,6 +886,7 @@ int gabe_set_scalars(gabe_ctx *c, const gabe_scalars *s)
     }
     d.t = s->t;
     d.nstep = s->nstep;
+    d.hist0 = s->nstep;   /* rows recorded before describe another trajectory */
     d.adot0_valid = 1;
     d.e0_valid = 1;
     d.error = 0;
@@ -1092,7 +1116,7 @@ int gabe_history(gabe_ctx *c, double *out, int max_rows)
     std::vector<double> h((size_t)c->hist_cap * GABE_HIST_COLS);
     HIPCHK(hipMemcpy(h.data(), c->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     const long total = (long)d.nstep;
-    const long avail = std::min<long>(total, c->hist_cap);
+    const long avail = std::min<long>(std::max<long long>(0, d.nstep - d.hist0), c->hist_cap);
     const long nrow = std::min<long>(avail, max_rows);
     for (long r = 0; r < nrow; r++) {
         const long stepno = total - nrow + r;
@@ -1344,8 +1368,11 @@ int gabe_ensemble_step(gabe_ensemble *e, int nsteps)
         int rc = ens_launch_step(e, p);
         if (rc == GABE_OK) rc = ens_launch_step(e, 1 - p);
         const hipError_t ec = hipStreamEndCapture(e->stream, &g);
-        if (rc != GABE_OK) return rc;
-        if (ec != hipSuccess) return fail(GABE_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        if (rc != GABE_OK || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);   /* a partial capture is not kept */
+            if (rc != GABE_OK) return rc;
+            return fail(GABE_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        }
         const hipError_t ei = hipGraphInstantiate(&e->graph[p], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ei != hipSuccess) return fail(GABE_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));

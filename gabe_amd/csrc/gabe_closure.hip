@@ -679,7 +679,7 @@ __global__ void k_gate(const DevScalars *sc, const double *hist, int hist_cap, d
 {
     if (threadIdx.x != 0 || blockIdx.x != 0 || gate[0] == 0) return;
     const long long n = sc->nstep - 1;
-    if (n < 2) return;
+    if (n - 2 < sc->hist0) return;   /* three rows of this trajectory are needed */
     const double *rm = hist + ((n - 2) % hist_cap) * GABE_HIST_COLS;
     const double *r0 = hist + ((n - 1) % hist_cap) * GABE_HIST_COLS;
     const double *rp = hist + (n % hist_cap) * GABE_HIST_COLS;
@@ -985,6 +985,41 @@ int restrict_array(gabe_ctx *f, gabe_ctx *c, void *src, void *dst)
 
 } // namespace
 
+namespace {
+
+/* (Re)attach a closure run to c from its state k and the packed weights and
+ * biases at pk, bias (device or host memory, as kind says). */
+int run_create(gabe_ctx *c, const gabe_detail::ClosureCkpt &k, const void *pk, const void *bias, hipMemcpyKind kind)
+{
+    if (c->closure) closure_release(c);
+    ClosureRun *R = new ClosureRun;
+    c->closure = R;
+    R->N = k.N;
+    R->thr = k.thr;
+    for (int q = 0; q < 4; q++) R->inv_s[q] = k.inv_s[q];
+    for (int q = 0; q < 2; q++) R->dt_s[q] = k.dt_s[q];
+    const long N3 = c->N3;
+    int rc = GABE_OK;
+    if (hipMalloc(&R->pk, sizeof(bf16x8) * PK_TOTAL) != hipSuccess || hipMalloc(&R->bias, sizeof(float) * 48) ||
+        hipMalloc(&R->a1, sizeof(bf16) * 16 * N3) ||
+        hipMalloc(&R->a2, sizeof(bf16) * 16 * N3) || hipMalloc(&R->u, sizeof(float) * 2 * N3) ||
+        hipMalloc(&R->w, sizeof(float) * 2 * N3) ||
+        hipMalloc(&R->gate, sizeof(int)) ||
+        hipMalloc(&R->fallback, sizeof(long long)) || hipMalloc(&R->rmax, sizeof(double)))
+        rc = fail(GABE_ERR_HIP, "closure attachment allocation failed");
+    if (rc == GABE_OK &&
+        (hipMemcpy(R->pk, pk, sizeof(bf16x8) * PK_TOTAL, kind) != hipSuccess ||
+         hipMemcpy(R->bias, bias, sizeof(float) * 48, kind) != hipSuccess ||
+         hipMemcpy(R->gate, &k.gate, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(R->fallback, &k.fallback, sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(R->rmax, &k.rmax, sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(GABE_ERR_HIP, "closure attachment copy failed");
+    if (rc != GABE_OK) closure_release(c);
+    return rc;
+}
+
+} // namespace
+
 namespace gabe_detail {
 
 int closure_after_step(gabe_ctx *c)
@@ -1029,6 +1064,49 @@ void closure_release(gabe_ctx *c)
         if (b) (void)hipFree(b);
     delete R;
     c->closure = nullptr;
+}
+
+size_t closure_ckpt_bytes(const gabe_ctx *c)
+{
+    return c->closure ? sizeof(ClosureCkpt) + sizeof(bf16x8) * PK_TOTAL + sizeof(float) * 48 : 0;
+}
+
+int closure_ckpt_save(gabe_ctx *c, char *p)
+{
+    ClosureRun *R = c->closure;
+    if (!R) return GABE_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ClosureCkpt k{};
+    k.N = R->N;
+    k.thr = R->thr;
+    for (int q = 0; q < 4; q++) k.inv_s[q] = R->inv_s[q];
+    for (int q = 0; q < 2; q++) k.dt_s[q] = R->dt_s[q];
+    HIPCHK(hipMemcpy(&k.gate, R->gate, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&k.fallback, R->fallback, sizeof(long long), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&k.rmax, R->rmax, sizeof(double), hipMemcpyDeviceToHost));
+    std::memcpy(p, &k, sizeof k);
+    p += sizeof k;
+    HIPCHK(hipMemcpy(p, R->pk, sizeof(bf16x8) * PK_TOTAL, hipMemcpyDeviceToHost));
+    p += sizeof(bf16x8) * PK_TOTAL;
+    HIPCHK(hipMemcpy(p, R->bias, sizeof(float) * 48, hipMemcpyDeviceToHost));
+    return GABE_OK;
+}
+
+int closure_ckpt_load(gabe_ctx *c, const char *p, size_t bytes)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (bytes == 0) {   /* the checkpointed lattice had no closure attached */
+        closure_release(c);
+        return GABE_OK;
+    }
+    if (bytes != sizeof(ClosureCkpt) + sizeof(bf16x8) * PK_TOTAL + sizeof(float) * 48)
+        return fail(GABE_ERR_ARG, "checkpoint's closure section has %zu bytes, this library writes %zu", bytes,
+                    sizeof(ClosureCkpt) + sizeof(bf16x8) * PK_TOTAL + sizeof(float) * 48);
+    ClosureCkpt k;
+    std::memcpy(&k, p, sizeof k);
+    if (k.N != c->N || !fp32_pair(c) || !c->own_stream)
+        return fail(GABE_ERR_ARG, "checkpoint carries a closure for N=%d that this lattice cannot take", k.N);
+    return run_create(c, k, p + sizeof k, p + sizeof k + sizeof(bf16x8) * PK_TOTAL, hipMemcpyHostToDevice);
 }
 
 } // namespace gabe_detail
@@ -1076,7 +1154,8 @@ int gabe_restrict(gabe_ctx *fine, gabe_ctx *coarse)
     CHK(read_scalars(fine, d));
     DevScalars dc;
     CHK(read_scalars(coarse, dc));
-    d.nstep = dc.nstep;    /* the coarse context keeps its own step count and history */
+    d.nstep = dc.nstep;    /* the coarse context keeps its own step count; its history */
+    d.hist0 = dc.nstep;    /* restarts with the restricted state */
     d.e0_valid = 0;
     CHK(write_scalars(coarse, d));
     return GABE_OK;
@@ -1316,34 +1395,15 @@ int gabe_closure_attach(gabe_ctx *c, gabe_closure *cl, double residual_threshold
     if (!(residual_threshold > 0.)) return fail(GABE_ERR_ARG, "residual threshold must be positive");
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipStreamSynchronize(cl->stream));
-    if (c->closure) closure_release(c);
-    ClosureRun *R = new ClosureRun;
-    c->closure = R;
-    R->N = cl->N;
-    R->thr = residual_threshold;
-    for (int q = 0; q < 4; q++) R->inv_s[q] = (float)(1. / cl->s_in[q]);
-    for (int q = 0; q < 2; q++) R->dt_s[q] = (float)(c->p.dt * cl->s_out[q]);
-    const long N3 = c->N3;
-    int rc = GABE_OK;
-    if (hipMalloc(&R->pk, sizeof(bf16x8) * PK_TOTAL) != hipSuccess || hipMalloc(&R->bias, sizeof(float) * 48) ||
-        hipMalloc(&R->a1, sizeof(bf16) * 16 * N3) ||
-        hipMalloc(&R->a2, sizeof(bf16) * 16 * N3) || hipMalloc(&R->u, sizeof(float) * 2 * N3) ||
-        hipMalloc(&R->w, sizeof(float) * 2 * N3) ||
-        hipMalloc(&R->gate, sizeof(int)) ||
-        hipMalloc(&R->fallback, sizeof(long long)) || hipMalloc(&R->rmax, sizeof(double)))
-        rc = fail(GABE_ERR_HIP, "closure attachment allocation failed");
-    const int one = 1;
-    const long long none = -1;
-    const double zero = 0.;
-    if (rc == GABE_OK &&
-        (hipMemcpy(R->pk, cl->pk, sizeof(bf16x8) * PK_TOTAL, hipMemcpyDeviceToDevice) != hipSuccess ||
-         hipMemcpy(R->bias, cl->bias, sizeof(float) * 48, hipMemcpyDeviceToDevice) != hipSuccess ||
-         hipMemcpy(R->gate, &one, sizeof one, hipMemcpyHostToDevice) != hipSuccess ||
-         hipMemcpy(R->fallback, &none, sizeof none, hipMemcpyHostToDevice) != hipSuccess ||
-         hipMemcpy(R->rmax, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess))
-        rc = fail(GABE_ERR_HIP, "closure attachment copy failed");
-    if (rc != GABE_OK) closure_release(c);
-    return rc;
+    ClosureCkpt k{};
+    k.N = cl->N;
+    k.gate = 1;
+    k.fallback = -1;
+    k.rmax = 0.;
+    k.thr = residual_threshold;
+    for (int q = 0; q < 4; q++) k.inv_s[q] = (float)(1. / cl->s_in[q]);
+    for (int q = 0; q < 2; q++) k.dt_s[q] = (float)(c->p.dt * cl->s_out[q]);
+    return run_create(c, k, cl->pk, cl->bias, hipMemcpyDeviceToDevice);
 }
 
 int gabe_closure_detach(gabe_ctx *c)

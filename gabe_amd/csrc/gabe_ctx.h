@@ -115,5 +115,16 @@ int write_scalars(gabe_ctx *c, const DevScalars &d);
 /* closure kernels after a step (gabe_closure.hip) and the attachment's teardown */
 int closure_after_step(gabe_ctx *c);
 void closure_release(gabe_ctx *c);
+/* the attached closure's part of a checkpoint: its gate (on/off), the step it
+ * switched off at, its largest residual, threshold, scales and weights */
+struct ClosureCkpt {
+    int N, gate;
+    long long fallback;
+    double rmax, thr;
+    float inv_s[4], dt_s[2];
+};
+size_t closure_ckpt_bytes(const gabe_ctx *c);            /* 0 without a closure */
+int closure_ckpt_save(gabe_ctx *c, char *p);
+int closure_ckpt_load(gabe_ctx *c, const char *p, size_t bytes);  /* 0 bytes: detach */
 
 } // namespace gabe_detail

@@ -27,6 +27,8 @@ struct DevScalars {
     int e0_valid;    /* ed*[0] belong to the current state */
     int error;       /* non-finite energy density seen */
     int pad;
+    long long hist0; /* first step whose history row belongs to the current trajectory
+                        (gabe_set_scalars, a checkpoint load or a restriction restart it) */
 };
 
 /* Values frozen by the step's finalize for sweep B and the GW mode kernel. */

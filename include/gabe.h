@@ -104,12 +104,16 @@ int gabe_means_vars(gabe_ctx *ctx, double *out);
 
 /* Checkpoint and resume (no counterpart in the reference, which has none).
  * The blob holds the parameters that fix the dynamics, the device scalars
- * (a, adot, energies, FP64 means, t, step count), the history rows in use and
- * the state (fields, velocities and, with tensor modes, h and l) in the
- * device's storage precision, so a context restored from it continues bit
- * for bit.  gabe_checkpoint_size gives the bytes gabe_checkpoint_save writes;
- * gabe_checkpoint_load accepts only a blob from a context with the same N,
- * nflds, precision, calc_gws, expansion_type, L, dt and model coefficients. */
+ * (a, adot, energies, FP64 means, t, step count), the history rows of the
+ * current trajectory, the state (fields, velocities and, with tensor modes,
+ * h and l) in the device's storage precision and, when a learned closure is
+ * attached, that attachment (on/off gate, the step it switched off at, its
+ * largest residual, threshold, scales and weights), so a context restored
+ * from it continues bit for bit.  gabe_checkpoint_size gives the bytes
+ * gabe_checkpoint_save writes; gabe_checkpoint_load accepts only a blob from
+ * a context with the same N, nflds, precision, calc_gws, expansion_type, L,
+ * dt, model coefficients and tensor-mode storage order, re-attaches the
+ * blob's closure (or detaches one the blob does not carry). */
 int gabe_checkpoint_size(gabe_ctx *ctx, size_t *bytes);
 int gabe_checkpoint_save(gabe_ctx *ctx, void *buf, size_t bytes);
 int gabe_checkpoint_load(gabe_ctx *ctx, const void *buf, size_t bytes);
@@ -125,7 +129,8 @@ int gabe_download_gw(gabe_ctx *ctx, double *h, double *l);
 
 /* Expansion and energy globals.  gabe_set_scalars marks adot[0] as belonging
  * to the current state (the first step then uses it, as step() does after
- * initexpansion(), g2init.cpp:26-29).  gabe_get_scalars returns the reference's
+ * initexpansion(), g2init.cpp:26-29) and starts a new trajectory: the
+ * history (gabe_history) holds only rows recorded after it.  gabe_get_scalars returns the reference's
  * post-step values: a[0], and edkin/edpot/edgrad/edrho[0] and adot[0] = adf(0)
  * of the current state (g2functions.cpp:268-270), computing them if stale. */
 int gabe_set_scalars(gabe_ctx *ctx, const gabe_scalars *s);
@@ -164,8 +169,10 @@ int gabe_effmass(gabe_ctx *ctx, int fld, double *out);
 int gabe_tt_source(gabe_ctx *ctx, int s, double *out);
 
 /* Per-step scalar history recorded on the device at each step's start:
- * rows of (t, a, adot, edkin, edpot, edgrad, edrho, step).  Returns the
- * number of rows written to out (at most max_rows, most recent last). */
+ * rows of (t, a, adot, edkin, edpot, edgrad, edrho, step), for the steps
+ * since the last gabe_set_scalars or restriction (a checkpoint load restores
+ * the blob's rows), at most the last 65536.  Returns the number of rows written to out (at most
+ * max_rows, most recent last). */
 int gabe_history(gabe_ctx *ctx, double *out, int max_rows);
 
 /* Timing hook: runs nsteps steps with HIP events around every kernel on the

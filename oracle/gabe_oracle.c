@@ -13,7 +13,7 @@
  *     FFTW_MEASURE clobbering of the T11 input (g2evolutionpert.cpp:183-187)
  *     is not reproduced;
  *   - the vacuum ICs draw rand() in the sequential loop order, i.e. the
- *     reference run with one thread (g2init.cpp:330-414).
+ *     reference run with one thread (the OpenMP loop of g2init.cpp:88-172).
  */
 #include "gabe_oracle.h"
 

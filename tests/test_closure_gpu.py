@@ -33,9 +33,11 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.fixture(scope="module")
-def net32():
-    N = 32
+@pytest.fixture(scope="module", params=[32, 128])
+def net(request):
+    """A random network with nonzero output biases and random inputs and
+    targets, at 32^3 and at the closure workload's 128^3."""
+    N = request.param
     cl = cm.Closure(N, 0.01, seed=5)
     p = cl.params().astype(np.float64)
     p[cr.NPARAMS - 2:] = [0.03, -0.02]   # nonzero output biases
@@ -47,8 +49,8 @@ def net32():
     cl.close()
 
 
-def test_forward_matches_cpu_network(net32):
-    cl, p, x, _ = net32
+def test_forward_matches_cpu_network(net):
+    cl, p, x, _ = net
     y = cl.forward(x.reshape(-1, 4)).reshape(x.shape[:3] + (2,))
     ref, _ = cr.forward(p, x, bf16=True)
     assert np.abs(y - ref).max() <= 1e-2 * np.abs(ref).max()
@@ -57,8 +59,8 @@ def test_forward_matches_cpu_network(net32):
     assert rel(y, ref64) < 5e-2
 
 
-def test_gradient_matches_cpu_network(net32):
-    cl, p, x, t = net32
+def test_gradient_matches_cpu_network(net):
+    cl, p, x, t = net
     loss, g = cl.gradient(x.reshape(-1, 4), t.reshape(2, -1))
     lref, gref = cr.loss_and_grad(p, x, t, bf16=True)
     assert abs(loss - lref) <= 1e-4 * lref
@@ -106,8 +108,8 @@ def test_restriction_keeps_the_coarse_band():
     assert sc["a"][0] == 1.25 and sc["t"] == 0.75
 
 
-def small_setup():
-    return cm.ClosureSetup.make(configs.lambda_phi4_config, 32, 4)
+def small_setup(N=32):
+    return cm.ClosureSetup.make(configs.lambda_phi4_config, N, 4)
 
 
 def test_zero_closure_is_the_plain_step_and_fallback_switches_it_off():
@@ -154,17 +156,17 @@ def stored_arrays(lat):
     return tail[:n].reshape(shp), tail[n:].reshape(shp)
 
 
-def test_step_correction_is_the_network_kick():
+@pytest.mark.parametrize("N", [32, 128])
+def test_step_correction_is_the_network_kick(N):
     """After one step with the closure attached the fields equal the plain
     step's, and the stored velocities differ by dt s_out[c] y_c / a^2: y the
     network output (the forward call, itself pinned to the CPU network above)
     for the post-step stored arrays (fluctuations about the FP64 means, as in
     the training samples) scaled by 1 / s_in and rounded to BF16; the step
     path reads the FP32 arrays inside its first convolution."""
-    setup = small_setup()
+    setup = small_setup(N)
     cfg = setup.coarse_cfg
-    assert cfg.calc_gws == 0
-    N = cfg.N
+    assert cfg.calc_gws == 0 and cfg.N == N
     A = host.initialized_lattice(cfg, precision=GABE_FP32)
     B = host.initialized_lattice(cfg, precision=GABE_FP32)
     f0, v0 = stored_arrays(B)
