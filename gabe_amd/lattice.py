@@ -26,7 +26,7 @@ KERNEL_CLASSES = ("sweep_a", "finalize", "stress_fft", "gw_modes", "sweep_b")
 
 EXPORTS = (
     "gabe_create", "gabe_destroy", "gabe_last_error", "gabe_numbins",
-    "gabe_upload_state", "gabe_download_state", "gabe_download_plane", "gabe_means_vars",
+    "gabe_upload_state", "gabe_download_state", "gabe_download_plane", "gabe_means_vars", "gabe_means_vars_noalloc",
     "gabe_upload_gw", "gabe_download_gw", "gabe_checkpoint_size", "gabe_checkpoint_save", "gabe_checkpoint_load",
     "gabe_set_scalars", "gabe_get_scalars", "gabe_energy", "gabe_step", "gabe_sync",
     "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
@@ -84,6 +84,7 @@ def load():
         "gabe_download_state": [vp, dp, dp],
         "gabe_download_plane": [vp, ip, dp, dp],
         "gabe_means_vars": [vp, dp],
+        "gabe_means_vars_noalloc": [vp, ip, C.c_longlong, ip, dp, vp, dp],
         "gabe_checkpoint_size": [vp, C.POINTER(C.c_size_t)],
         "gabe_checkpoint_save": [vp, vp, C.c_size_t],
         "gabe_checkpoint_load": [vp, vp, C.c_size_t],
@@ -154,6 +155,17 @@ def make_params(N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1, preci
         dt = L / N / 20.0
     return GabeParams(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
                       gw_spectrum_quirk, planes_per_block)
+
+
+def means_vars_noalloc(dev_ptr, precision, nsites, nflds, base=None, stream=None):
+    """gabe_means_vars_noalloc on fields at device address dev_ptr (nflds
+    arrays of nsites doubles or floats); stream is a hipStream_t handle
+    (None: the null stream).  Returns [mean, variance] per field."""
+    out = np.zeros(2 * nflds)
+    b = None if base is None else np.ascontiguousarray(base, dtype=np.float64)
+    _check(load().gabe_means_vars_noalloc(C.c_void_p(dev_ptr), precision, nsites, nflds,
+                                           None if b is None else _dp(b), C.c_void_p(stream or 0), _dp(out)))
+    return out
 
 
 def autotune_sweep(**kw):

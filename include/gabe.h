@@ -102,6 +102,23 @@ int gabe_download_plane(gabe_ctx *ctx, int i, double *field, double *dfield);
  * first site's value as the shift of a one-pass sum. */
 int gabe_means_vars(gabe_ctx *ctx, double *out);
 
+/* gabe_means_vars on fields in caller-owned device memory, allocating
+ * nothing (no device or host heap memory, no library routine that
+ * allocates).  No counterpart in the reference; it is for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  field holds nflds arrays of nsites values each,
+ * back to back, as doubles (GABE_FP64) or floats (GABE_FP32); base[fld]
+ * (NULL: 0) is added to each mean (the FP64 background of a fluctuation
+ * array).  out[2*fld] = mean, out[2*fld+1] = variance, the shifted one-pass
+ * reduction of gabe_means_vars.  stream is a hipStream_t (NULL: the null
+ * stream); the call synchronises it.  1 <= nsites <= GABE_NOALLOC_MAX_SITES.
+ * The partial sums use a fixed 16 KiB device array inside the library, one
+ * copy per GPU, shared by every call on that GPU: calls that overlap on one
+ * GPU race. */
+#define GABE_NOALLOC_MAX_SITES (1LL << 33)
+int gabe_means_vars_noalloc(const void *field, int precision, long long nsites, int nflds, const double *base,
+                            void *stream, double *out);
+
 /* Checkpoint and resume (no counterpart in the reference, which has none).
  * The blob holds the parameters that fix the dynamics, the device scalars
  * (a, adot, energies, FP64 means, t, step count), the history rows of the
