@@ -77,10 +77,13 @@ def read_traffic(kernel, N):
 
 def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
     """Time `steps` steps after `warmup`; then `profile_steps` more with
-    per-kernel events.  Returns the time, kernel means and the final state,
-    scalars, spectra and Friedmann residual for the precision comparison."""
+    per-kernel events.  Returns the time, kernel means, the tiling in use and
+    the final state, scalars, spectra and Friedmann residual for the precision
+    comparison."""
     from gabe_amd import host
     lat = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb)
+    q = lat.query()
+    tiling = {"planes_per_block": q.planes_per_block, "transform_tile": q.transform_tile, "mode_tile": q.mode_tile}
     lat.step(warmup)
     lat.sync()
     t0 = time.perf_counter()
@@ -89,7 +92,7 @@ def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
     t1 = time.perf_counter()
     ktimes = lat.kernel_times(profile_steps) if profile_steps else {}
     final = {"scalars": lat.get_scalars(), "state": lat.download_state(), "spectra": lat.specOut(),
-             "friedmann_max": host.friedmann_residual(lat.history())[0]}
+             "friedmann_max": host.friedmann_residual(lat.history())[0], "tiling": tiling}
     lat.close()
     return t1 - t0, ktimes, final
 
@@ -303,7 +306,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 64-lattice ensemble (0: skip)")
     ap.add_argument("--closure-steps", type=int, default=200, help="steps of the 128^3 closure timing (0: skip)")
-    ap.add_argument("--ppb", type=int, default=0, help="lattice planes per sweep workgroup (0: library default)")
+    ap.add_argument("--ppb", type=int, default=0, help="lattice planes per sweep workgroup (0: autotuned)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the built-in default tiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs every lattice on a single MI355X "
@@ -317,15 +321,27 @@ def main():
     cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
     sites = N ** 3
 
+    autotune = None
+    if not args.no_autotune:
+        from gabe_amd.lattice import autotune_sweep
+        autotune = {}
+        for name, prec in (("fp32", GABE_FP32), ("fp64", GABE_FP64)):
+            if name == "fp64" and args.no_fp64:
+                continue
+            print(f"[bench] autotuning the {name} {N}^3 tiles", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            ppb, ms = autotune_sweep(precision=prec, **cfg.lattice_kwargs())
+            autotune[name] = {"best_ms_per_step": ms, "seconds": time.perf_counter() - t0}
     print(f"[bench] FP32 {N}^3: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     dt32, k32, fin32 = run_device(cfg, GABE_FP32, args.steps, args.warmup, args.profile_steps, args.ppb)
-    sc32, state = fin32["scalars"], fin32["state"]
+    sc32, state, tiling32 = fin32["scalars"], fin32["state"], fin32["tiling"]
     rate32 = sites * args.steps / dt32
     fp64 = None
     if not args.no_fp64:
         print(f"[bench] FP64 {N}^3", file=sys.stderr, flush=True)
         dt64, k64, fin64 = run_device(cfg, GABE_FP64, args.steps, args.warmup, args.profile_steps, args.ppb)
         fp64 = {"value": sites * args.steps / dt64, "ms_per_step": 1e3 * dt64 / args.steps,
+                "tiling": fin64["tiling"],
                 "kernels_ms": k64,
                 "step_hbm_frac": b_alg(8, N) * sites * args.steps / dt64 / (HBM_PEAK_GBS * 1e9),
                 "fp32_vs_fp64": dict(steps=args.warmup + args.steps + args.profile_steps,
@@ -380,6 +396,9 @@ def main():
         "roofline": roofline,
         "step_hbm_frac": b_alg(4, N) * rate32 / (HBM_PEAK_GBS * 1e9),
         "step_bytes_per_site": {"b_alg": b_alg(4, N), "this_pipeline": sum(ab.values())},
+        "step_frac_own_bytes": sum(ab.values()) * rate32 / (HBM_PEAK_GBS * 1e9),
+        "tiling": tiling32,
+        "autotune": autotune,
         "kernels": kernels,
         "fp64": fp64,
         "ensemble": ensemble,

@@ -233,12 +233,13 @@ int gw_modes(gabe_ctx *c, hipStream_t st = nullptr)
     const double dx3 = c->dx * c->dx * c->dx;
     if (c->fp32) {
         GwFftArgs<float> g{(const float *)c->shat, (float *)c->h, (float *)c->l, c->ss, (const float *)c->twiddle,
-                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4, (const float *)nyq_plane(c)};
+                           c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4, (const float *)nyq_plane(c),
+                           c->gw_tile};
         launch_gw_ifft<float>(g, st);
     } else {
         GwFftArgs<double> g{(const double *)c->shat, (double *)c->h, (double *)c->l, c->ss,
                             (const double *)c->twiddle, c->N, c->M, c->kpad, c->p.L, c->p.dt, dx3, c->lay.nt4,
-                            (const double *)nyq_plane(c)};
+                            (const double *)nyq_plane(c), c->gw_tile};
         launch_gw_ifft<double>(g, st);
     }
     HIPCHK(hipGetLastError());
@@ -333,17 +334,22 @@ int refresh_e0(gabe_ctx *c)
     return GABE_OK;
 }
 
-/* sweep tiling chosen by gabe_autotune_sweep, keyed by (N, nflds, FP32) */
-std::map<long, int> g_tuned;
+/* tiling chosen by gabe_autotune_sweep, keyed by (N, nflds, FP32, tensor modes) */
+struct Tiling {
+    int planes = 0;      /* sweep planes per workgroup */
+    int cw = 0;          /* j-transform columns per workgroup */
+    int mode_tile = 0;   /* fused i-transform + mode kernel variant */
+};
+std::map<long, Tiling> g_tuned;
 std::mutex g_tuned_mu;
 
-long tune_key(int N, int nf, bool fp32) { return ((long)N * 4 + nf) * 2 + (fp32 ? 1 : 0); }
+long tune_key(int N, int nf, bool fp32, bool gw) { return (((long)N * 4 + nf) * 2 + (fp32 ? 1 : 0)) * 2 + (gw ? 1 : 0); }
 
-int tuned_planes(int N, int nf, bool fp32)
+Tiling tuned_tiling(int N, int nf, bool fp32, bool gw)
 {
     std::lock_guard<std::mutex> lock(g_tuned_mu);
-    const auto it = g_tuned.find(tune_key(N, nf, fp32));
-    return it == g_tuned.end() ? 0 : it->second;
+    const auto it = g_tuned.find(tune_key(N, nf, fp32, gw));
+    return it == g_tuned.end() ? Tiling{} : it->second;
 }
 
 int check_params(const gabe_params *p)
@@ -355,6 +361,11 @@ int check_params(const gabe_params *p)
     if (p->expansion_type < 0 || p->expansion_type > 2) return fail(GABE_ERR_ARG, "expansion_type must be 0, 1 or 2");
     if (p->precision != GABE_FP64 && p->precision != GABE_FP32) return fail(GABE_ERR_ARG, "precision must be GABE_FP64 or GABE_FP32");
     if ((long)p->N * p->N * p->N > (1L << 31)) return fail(GABE_ERR_ARG, "N^3 beyond 2^31 is not supported");
+    if (p->transform_tile != 0 && p->transform_tile != 16 && p->transform_tile != 32)
+        return fail(GABE_ERR_ARG, "transform_tile must be 0, 16 or 32 (got %d)", p->transform_tile);
+    const int nt = gabe_gwfft_tiles(p->N, p->precision == GABE_FP32);
+    if (p->mode_tile < 0 || p->mode_tile > std::max(nt, 1))
+        return fail(GABE_ERR_ARG, "mode_tile must be 0..%d at N=%d (got %d)", std::max(nt, 1), p->N, p->mode_tile);
     return GABE_OK;
 }
 
@@ -410,7 +421,11 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->lay.nt4 = (int)((c->lay.KP + 3) / 4);
     c->lay.tiled = c->gwfft ? 1 : 0;
     c->mslots = c->lay.tiled ? (long)c->N * c->lay.nt4 * c->N * 4 : c->M;
-    if (const char *e = std::getenv("GABE_FFT_TILE")) c->fft_cw = std::atoi(e) == 32 ? 32 : 16;
+    {
+        const Tiling t = tuned_tiling(c->N, c->nf, c->fp32, c->alloc_gw);
+        c->fft_cw = params->transform_tile ? params->transform_tile : (t.cw ? t.cw : 16);
+        c->gw_tile = params->mode_tile ? params->mode_tile : (t.mode_tile ? t.mode_tile : 1);
+    }
     HIPCHK(hipMalloc(&c->stress, c->esz * narr * c->N3));
     const size_t shat_bytes = 2 * c->esz * narr * std::max(c->M, c->N2 * c->kpad);
     HIPCHK(hipMalloc(&c->shat, shat_bytes));
@@ -433,7 +448,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     /* sweep tiling: one thread per plane column, ci planes per workgroup */
     const int bx = (int)((c->N2 + GABE_BLOCK - 1) / GABE_BLOCK);
     int ci = params->planes_per_block;
-    if (ci <= 0) ci = tuned_planes(c->N, c->nf, c->fp32);
+    if (ci <= 0) ci = tuned_tiling(c->N, c->nf, c->fp32, c->alloc_gw).planes;
     if (ci <= 0) {
         const int by_target = std::max(1, 2048 / bx);
         ci = (c->N + by_target - 1) / by_target;
@@ -1280,22 +1295,28 @@ int gabe_query(gabe_ctx *c, gabe_params *out)
     if (!c || !out) return fail(GABE_ERR_ARG, "NULL argument");
     *out = c->p;
     out->planes_per_block = c->ci;
+    out->transform_tile = c->colfft ? c->fft_cw : 0;
+    out->mode_tile = c->gwfft ? c->gw_tile : 0;
     return GABE_OK;
 }
+
+int gabe_mode_tiles(int N, int precision) { return gabe_gwfft_tiles(N, precision == GABE_FP32); }
 
 int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double *ms_per_step)
 {
     CHK(check_params(params));
     gabe_params p = *params;
-    const int cands[] = {1, 2, 4, 8, 16, 32, 64};
-    int best = 0;
-    float best_ms = 0.f;
-    for (int ci : cands) {
-        if (ci > p.N) break;
-        p.planes_per_block = ci;
+    const bool fp32 = p.precision == GABE_FP32, gw = p.calc_gws != 0;
+    /* time 10 steps (after 3) of a context with tiling t; ms < 0 if the
+     * context does not use the fused transform path (so no transform tiles) */
+    auto trial = [&](const Tiling &t, float &ms, bool &fused) -> int {
+        p.planes_per_block = t.planes;
+        p.transform_tile = t.cw;
+        p.mode_tile = t.mode_tile;
         gabe_ctx *c = new gabe_ctx;
         int rc = create_impl(&p, c);
         if (rc == GABE_OK) {
+            fused = c->gwfft;
             /* a smooth synthetic state: timings do not depend on the values */
             const long n = (long)p.nflds * c->N3;
             std::vector<double> f(n), d(n);
@@ -1304,23 +1325,42 @@ int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double
                 d[q] = -0.2 + 1e-6 * std::cos(0.013 * (double)q);
             }
             rc = gabe_upload_state(c, f.data(), d.data());
-            float ms = 0.f;
             if (rc == GABE_OK) rc = time_steps(c, 3, ms);
             if (rc == GABE_OK) rc = time_steps(c, 10, ms);
-            if (rc == GABE_OK && (best == 0 || ms < best_ms)) {
-                best = ci;
-                best_ms = ms;
-            }
         }
         destroy_impl(c);
         delete c;
-        if (rc != GABE_OK) return rc;
+        return rc;
+    };
+    Tiling best;
+    float best_ms = 0.f;
+    bool fused = false;
+    for (int ci : {1, 2, 4, 8, 16, 32, 64}) {
+        if (ci > p.N) break;
+        float ms = 0.f;
+        CHK(trial(Tiling{ci, 16, 1}, ms, fused));
+        if (best.planes == 0 || ms < best_ms) {
+            best = Tiling{ci, 16, 1};
+            best_ms = ms;
+        }
+    }
+    if (gw && fused) {   /* the transform tiles at the chosen sweep tiling */
+        const int nt = gabe_gwfft_tiles(p.N, fp32);
+        for (int cw : {16, 32})
+            for (int mt = 1; mt <= nt; mt++) {
+                float ms = 0.f;
+                CHK(trial(Tiling{best.planes, cw, mt}, ms, fused));
+                if (ms < best_ms) {
+                    best = Tiling{best.planes, cw, mt};
+                    best_ms = ms;
+                }
+            }
     }
     {
         std::lock_guard<std::mutex> lock(g_tuned_mu);
-        g_tuned[tune_key(p.N, p.nflds, p.precision == GABE_FP32)] = best;
+        g_tuned[tune_key(p.N, p.nflds, fp32, gw)] = best;
     }
-    if (planes_per_block) *planes_per_block = best;
+    if (planes_per_block) *planes_per_block = best.planes;
     if (ms_per_step) *ms_per_step = best_ms / 10.;
     return GABE_OK;
 }

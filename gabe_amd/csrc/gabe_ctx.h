@@ -93,6 +93,7 @@ struct gabe_ctx {
     bool rowfft = false;         /* k rows transformed inside sweep A */
     long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
     int fft_cw = 16;             /* column tile width of the j/i transforms */
+    int gw_tile = 1;             /* tile variant of the fused i-transform + mode kernel */
     void *twiddle = nullptr;     /* exp(-2 pi i e/N), e < N, complex storage type */
     void *fft_work = nullptr;
     size_t fft_work_size = 0;

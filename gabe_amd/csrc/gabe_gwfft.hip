@@ -351,6 +351,21 @@ void launch_cfg(const GwFftArgs<T> &a, hipStream_t s)
 
 bool gabe_gwfft_supported(int N, bool fp32) { return N >= 16 && N <= (fp32 ? 512 : 256) && (N & (N - 1)) == 0; }
 
+/* Tile variants (kz columns per workgroup CW, components per round CPR,
+ * threads NT) at the sizes where more than one was worth keeping; variant 1
+ * is the built-in default, the rest are the alternatives the autotuner
+ * times.  At 256^3 FP32 the default (4 columns, 3 components per round, 256
+ * threads: 3 workgroups per CU) measured fastest once the mode loads moved
+ * into the last round; at 512^3 the 512-thread tile shares the CUs better
+ * with the sweeps (a 7 % shorter pipelined step than 2 x 6 on 256 threads at
+ * the same kernel time). */
+int gabe_gwfft_tiles(int N, bool fp32)
+{
+    if (!gabe_gwfft_supported(N, fp32)) return 0;
+    if (fp32) return N == 256 ? 7 : N == 512 ? 4 : 1;
+    return N == 256 ? 3 : 1;
+}
+
 template <>
 void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
 {
@@ -359,35 +374,25 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
     case 32: launch_cfg<float, 8, 4, 8, 6, 256>(a, s); break;
     case 64: launch_cfg<float, 8, 8, 8, 6, 512>(a, s); break;
     case 128: launch_cfg<float, 16, 8, 8, 6, 1024>(a, s); break;
-    case 256: {
-        /* tile variants (kz columns per workgroup, components per round,
-         * threads); the default, 4 columns and 3 components per round on 256
-         * threads (3 workgroups per CU), measured fastest once the mode loads
-         * moved into the last round; 4 columns x 6 components on 512 threads
-         * was faster before that */
-        const char *v = std::getenv("GABE_GWFFT_TILE");
-        const int t = v ? std::atoi(v) : 0;
-        if (t == 1) launch_cfg<float, 16, 16, 4, 6, 512>(a, s);
-        else if (t == 2) launch_cfg<float, 16, 16, 4, 6, 256>(a, s);
-        else if (t == 3) launch_cfg<float, 16, 16, 8, 6, 512>(a, s);
-        else if (t == 4) launch_cfg<float, 16, 16, 8, 3, 512>(a, s);
-        else if (t == 5) launch_cfg<float, 16, 16, 4, 3, 512>(a, s);
-        else if (t == 6) launch_cfg<float, 16, 16, 8, 6, 1024>(a, s);
-        else launch_cfg<float, 16, 16, 4, 3, 256>(a, s);
+    case 256:
+        switch (a.tile) {
+        case 2: launch_cfg<float, 16, 16, 4, 6, 512>(a, s); break;
+        case 3: launch_cfg<float, 16, 16, 4, 6, 256>(a, s); break;
+        case 4: launch_cfg<float, 16, 16, 8, 6, 512>(a, s); break;
+        case 5: launch_cfg<float, 16, 16, 8, 3, 512>(a, s); break;
+        case 6: launch_cfg<float, 16, 16, 4, 3, 512>(a, s); break;
+        case 7: launch_cfg<float, 16, 16, 8, 6, 1024>(a, s); break;
+        default: launch_cfg<float, 16, 16, 4, 3, 256>(a, s); break;
+        }
         break;
-    }
-    case 512: {
-        const char *v = std::getenv("GABE_GWFFT_TILE");
-        const int t = v ? std::atoi(v) : 0;
-        /* 4 columns x 3 components on 512 threads: the same kernel time as 2
-         * columns x 6 components on 256 threads, but the pipelined 512^3 step
-         * is 7 % shorter with it (it shares the CUs better with the sweeps) */
-        if (t == 1) launch_cfg<float, 32, 16, 2, 6, 256>(a, s);
-        else if (t == 2) launch_cfg<float, 32, 16, 2, 6, 512>(a, s);
-        else if (t == 3) launch_cfg<float, 32, 16, 2, 3, 256>(a, s);
-        else launch_cfg<float, 32, 16, 4, 3, 512>(a, s);
+    case 512:
+        switch (a.tile) {
+        case 2: launch_cfg<float, 32, 16, 2, 6, 256>(a, s); break;
+        case 3: launch_cfg<float, 32, 16, 2, 6, 512>(a, s); break;
+        case 4: launch_cfg<float, 32, 16, 2, 3, 256>(a, s); break;
+        default: launch_cfg<float, 32, 16, 4, 3, 512>(a, s); break;
+        }
         break;
-    }
     default: break;
     }
 }
@@ -397,17 +402,17 @@ void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
 template <>
 void launch_gw_ifft<double>(const GwFftArgs<double> &a, hipStream_t s)
 {
-    const char *v = std::getenv("GABE_GWFFT_TILE");
-    const int t = v ? std::atoi(v) : 0;
     switch (a.N) {
     case 16: launch_cfg<double, 4, 4, 8, 6, 256>(a, s); break;
     case 32: launch_cfg<double, 8, 4, 8, 6, 256>(a, s); break;
     case 64: launch_cfg<double, 8, 8, 4, 6, 256>(a, s); break;
     case 128: launch_cfg<double, 16, 8, 4, 6, 512>(a, s); break;
     case 256:
-        if (t == 1) launch_cfg<double, 16, 16, 2, 3, 256>(a, s);
-        else if (t == 2) launch_cfg<double, 16, 16, 4, 3, 256>(a, s);
-        else launch_cfg<double, 16, 16, 2, 6, 256>(a, s);
+        switch (a.tile) {
+        case 2: launch_cfg<double, 16, 16, 2, 3, 256>(a, s); break;
+        case 3: launch_cfg<double, 16, 16, 4, 3, 256>(a, s); break;
+        default: launch_cfg<double, 16, 16, 2, 6, 256>(a, s); break;
+        }
         break;
     default: break;
     }

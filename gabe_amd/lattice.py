@@ -30,7 +30,7 @@ EXPORTS = (
     "gabe_upload_gw", "gabe_download_gw", "gabe_checkpoint_size", "gabe_checkpoint_save", "gabe_checkpoint_load",
     "gabe_set_scalars", "gabe_get_scalars", "gabe_energy", "gabe_step", "gabe_sync",
     "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
-    "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep",
+    "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep", "gabe_mode_tiles",
     "gabe_ensemble_create", "gabe_ensemble_member", "gabe_ensemble_step", "gabe_ensemble_sync",
     "gabe_ensemble_destroy",
     "gabe_restrict", "gabe_closure_create", "gabe_closure_destroy", "gabe_closure_params",
@@ -45,7 +45,8 @@ class GabeParams(C.Structure):
         ("N", C.c_int), ("nflds", C.c_int), ("L", C.c_double), ("dt", C.c_double),
         ("calc_gws", C.c_int), ("expansion_type", C.c_int), ("precision", C.c_int),
         ("m2_phi", C.c_double), ("lam", C.c_double), ("g2", C.c_double), ("m2_chi", C.c_double),
-        ("gw_spectrum_quirk", C.c_int), ("planes_per_block", C.c_int), ("reserved", C.c_int * 6),
+        ("gw_spectrum_quirk", C.c_int), ("planes_per_block", C.c_int), ("transform_tile", C.c_int),
+        ("mode_tile", C.c_int), ("reserved", C.c_int * 4),
     ]
 
 
@@ -103,6 +104,7 @@ def load():
         "gabe_kernel_times": [vp, ip, dp],
         "gabe_query": [vp, C.POINTER(GabeParams)],
         "gabe_autotune_sweep": [C.POINTER(GabeParams), C.POINTER(C.c_int), dp],
+        "gabe_mode_tiles": [ip, ip],
         "gabe_ensemble_create": [C.POINTER(GabeParams), ip, C.POINTER(vp)],
         "gabe_ensemble_member": [vp, ip, C.POINTER(vp)],
         "gabe_ensemble_step": [vp, ip],
@@ -150,11 +152,12 @@ def numbins(N):
 
 
 def make_params(N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1, precision=GABE_FP64, m2_phi=1.0,
-                lam=0.0, g2=0.0, m2_chi=0.0, gw_spectrum_quirk=1, planes_per_block=0):
+                lam=0.0, g2=0.0, m2_chi=0.0, gw_spectrum_quirk=1, planes_per_block=0, transform_tile=0,
+                mode_tile=0):
     if dt is None:
         dt = L / N / 20.0
     return GabeParams(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
-                      gw_spectrum_quirk, planes_per_block)
+                      gw_spectrum_quirk, planes_per_block, transform_tile, mode_tile)
 
 
 def means_vars_noalloc(dev_ptr, precision, nsites, nflds, base=None, stream=None):
@@ -168,9 +171,16 @@ def means_vars_noalloc(dev_ptr, precision, nsites, nflds, base=None, stream=None
     return out
 
 
+def mode_tiles(N, precision):
+    """number of tile variants of the fused i-transform + mode kernel"""
+    return load().gabe_mode_tiles(N, precision)
+
+
 def autotune_sweep(**kw):
-    """Time the candidate sweep tilings for these parameters; later lattices
-    of the same (N, nflds, precision) use the fastest.  Returns (ppb, ms)."""
+    """Time the candidate sweep tilings for these parameters (and, with
+    tensor modes, the j-transform widths and mode-kernel variants); later
+    lattices of the same (N, nflds, precision, calc_gws) use the fastest.
+    Returns (planes per workgroup, ms per step)."""
     p = make_params(**kw)
     ppb = C.c_int()
     ms = C.c_double()
@@ -187,9 +197,9 @@ class Lattice:
 
     def __init__(self, N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1,
                  precision=GABE_FP64, m2_phi=1.0, lam=0.0, g2=0.0, m2_chi=0.0,
-                 gw_spectrum_quirk=1, planes_per_block=0, _handle=None):
+                 gw_spectrum_quirk=1, planes_per_block=0, transform_tile=0, mode_tile=0, _handle=None):
         self.params = make_params(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
-                                  gw_spectrum_quirk, planes_per_block)
+                                  gw_spectrum_quirk, planes_per_block, transform_tile, mode_tile)
         self.N, self.nflds, self.calc_gws = N, nflds, calc_gws
         self.M = N * N * (N // 2 + 1)
         self._owned = _handle is None

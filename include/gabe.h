@@ -63,8 +63,13 @@ typedef struct gabe_params {
     int gw_spectrum_quirk; /* 1: GW spectrum kz=0 term read from the kz=N/2 plane,
                               as g2spectra.cpp:149-164 does; 0: read kz=0         */
     int planes_per_block;  /* sweep tiling: lattice planes each workgroup marches
-                              through (0 = library default)                        */
-    int reserved[6];
+                              through (0 = autotuned or library default)           */
+    int transform_tile;    /* j-transform of the stress spectra: columns per
+                              workgroup, 16 or 32 (0 = autotuned or 16)            */
+    int mode_tile;         /* tile variant of the fused i-transform + mode kernel,
+                              1..gabe_mode_tiles(N, precision) (0 = autotuned or
+                              variant 1, the built-in default)                     */
+    int reserved[4];
 } gabe_params;
 
 /* The reference's expansion/energy globals (g2header.h:59-64) plus bookkeeping. */
@@ -203,11 +208,18 @@ int gabe_kernel_times(gabe_ctx *ctx, int nsteps, double *ms_out);
  * in planes_per_block.  No reference counterpart (introspection). */
 int gabe_query(gabe_ctx *ctx, gabe_params *out);
 
-/* Sweep-tiling autotuner: times a few steps of a synthetic lattice of these
- * parameters for each candidate planes_per_block and remembers the fastest
- * for (N, nflds, precision); later contexts created with planes_per_block = 0
- * use it.  Either output pointer may be NULL.  No reference counterpart. */
+/* Tiling autotuner: times a few steps of a synthetic lattice of these
+ * parameters for each candidate planes_per_block; with tensor modes on the
+ * fused transform path it then times, at the best planes_per_block, every
+ * j-transform tile width and mode-kernel tile variant.  The fastest choice is
+ * remembered for (N, nflds, precision, calc_gws); later contexts created with
+ * 0 in those fields use it (gabe_query reports what a context uses).  Either
+ * output pointer may be NULL; ms_per_step is the best step time.  No
+ * reference counterpart. */
 int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double *ms_per_step);
+/* Number of mode-kernel tile variants for this lattice side and precision
+ * (1 where only the default exists, 0 where the fused kernel does not run). */
+int gabe_mode_tiles(int N, int precision);
 
 /* Ensembles: independent lattices (sizes, couplings, seeds and a(t) of their
  * own) advanced together, each RK2 phase of every member in one launch and

@@ -73,3 +73,15 @@ def test_autotuner_choice_is_used():
     assert ppb in (1, 2, 4, 8, 16, 32) and ms > 0
     lat = Lattice(N=32, nflds=2, L=20.0, precision=GABE_FP32, lam=8.5, g2=855.0)
     assert lat.query().planes_per_block == ppb
+
+
+def test_autotuner_picks_transform_tiles_with_tensor_modes():
+    """With tensor modes the autotuner also times the j-transform widths and
+    the mode-kernel variants; a later context uses its choice."""
+    from gabe_amd.lattice import mode_tiles
+    kw = dict(N=32, nflds=2, L=20.0, precision=GABE_FP32, lam=8.5, g2=855.0, calc_gws=1)
+    ppb, ms = autotune_sweep(**kw)
+    assert ms > 0
+    q = Lattice(**kw).query()
+    assert q.planes_per_block == ppb and q.transform_tile in (16, 32)
+    assert 1 <= q.mode_tile <= mode_tiles(32, GABE_FP32)

@@ -385,3 +385,42 @@ def test_odd_plane_count_at_256():
         assert np.linalg.norm(f1[f] - f0[f]) <= 1e-6 * np.linalg.norm(fl)
     ref.close()
     lat.close()
+
+
+@pytest.mark.parametrize("precision", [GABE_FP32, GABE_FP64])
+def test_transform_and_mode_tiles_agree_at_256(precision):
+    """Every tile variant of the fused i-transform + mode kernel, and the
+    32-column j transform, give the tensor modes of the default tiling (the
+    one checked against the oracle at 256^3) to rounding: the variants only
+    regroup the same transforms and mode updates over workgroups."""
+    from gabe_amd import host
+    from gabe_amd.lattice import mode_tiles
+    cfg = configs.lambda_phi4_config(N=256, calc_gws=1)
+    ref = host.initialized_lattice(cfg, precision=precision, transform_tile=16, mode_tile=1)
+    ref.step(2)
+    h0, l0 = ref.download_gw()
+    f0, _ = ref.download_state()
+    ref.close()
+    tol = 1e-6 if precision == GABE_FP32 else 1e-13
+    n = mode_tiles(256, precision)
+    assert n == (7 if precision == GABE_FP32 else 3)
+    for mt, tt in [(m, 16) for m in range(2, n + 1)] + [(1, 32)]:
+        lat = host.initialized_lattice(cfg, precision=precision, transform_tile=tt, mode_tile=mt)
+        q = lat.query()
+        assert (q.mode_tile, q.transform_tile) == (mt, tt)
+        lat.step(2)
+        h, l = lat.download_gw()
+        print(f"mode tile {mt}, transform tile {tt}: h {rel_err(h, h0):.1e} l {rel_err(l, l0):.1e}", flush=True)
+        assert rel_err(h, h0) < tol and rel_err(l, l0) < tol, (mt, tt)
+        f, _ = lat.download_state()
+        assert np.array_equal(f, f0)   # the field path does not depend on these tiles
+        lat.close()
+
+
+def test_tile_parameters_are_validated():
+    cfg = configs.lambda_phi4_config(N=32, calc_gws=1)
+    from gabe_amd.lattice import GabeError, Lattice
+    with pytest.raises(GabeError):
+        Lattice(precision=GABE_FP32, transform_tile=8, **cfg.lattice_kwargs())
+    with pytest.raises(GabeError):
+        Lattice(precision=GABE_FP32, mode_tile=2, **cfg.lattice_kwargs())
