@@ -20,7 +20,11 @@
 #include "gabe.h"
 #include "gabe_host.h"
 #include "gabe_model.h"
+/* the run's parameters in GABE's own file format, then its model plug-in;
+ * the build's include path picks the directory holding them (params/...) */
 #include "g2parameters.h"
+#include "g2switches.h"
+#include "g2model.h"
 
 namespace {
 

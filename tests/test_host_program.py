@@ -1,9 +1,13 @@
 """The kept host program (gabe_amd/host/gabe_main.cpp) end to end: its output
 files against the oracle driven through the reference's cadence.
 
-The test builds are the default model at 16^3 with tensor modes, FP64, 12
-steps and a slice every 4 calls: gabe_test16 with the reference's default
-switches (random modes, plane slices), gabe_test16_sym3d with symmetric modes
+The test builds run 16^3 with tensor modes, FP64, 12 steps and a slice every
+4 calls.  gabe_ref16 is GABE's shipped parameter file (same names and
+spelling, #define switches included) with a user's edits for that short run,
+under GABE's own model (m^2 phi^2/2 + g^2 phi^2 chi^2/2): the drop-in of a
+GABE parameter file.  The others use the benchmark model (lambda phi^4 +
+g^2 phi^2 chi^2): gabe_test16 with the reference's default switches (random
+modes, plane slices), gabe_test16_sym3d with symmetric modes
 (field_full_rand 0), the sqrt(3)/8 window and whole-lattice slices
 (field_outdim 3), gabe_test16_cut1d with windowed random modes and line
 slices (field_outdim 1).  Expected values come from the
@@ -31,15 +35,19 @@ from tests.helpers import seeded_oracle
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 N, STEPS, SLICEWAIT = 16, 12, 4
 CUT = math.sqrt(3.0) / 8.0
-BUILDS = {   # executable: (field_full_rand, spec_cut_off, field_outdim)
-    "gabe_test16": (1, 0.0, 2),
-    "gabe_test16_sym3d": (0, CUT, 3),
-    "gabe_test16_cut1d": (1, CUT, 1),
+BUILDS = {   # executable: (field_full_rand, spec_cut_off, field_outdim, model)
+    "gabe_test16": (1, 0.0, 2, "lambda_phi4"),
+    "gabe_test16_sym3d": (0, CUT, 3, "lambda_phi4"),
+    "gabe_test16_cut1d": (1, CUT, 1, "lambda_phi4"),
+    "gabe_ref16": (1, 0.0, 2, "reference_default"),
 }
 
 
-def expected_outputs(full_rand, cut, outdim):
-    cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+def expected_outputs(full_rand, cut, outdim, model):
+    if model == "lambda_phi4":
+        cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+    else:   # GABE's shipped model and parameters (g2parameters.h:43-62), N = 16
+        cfg = configs.reference_default_config(N=N, calc_gws=1)
     cfg.field_full_rand = full_rand
     cfg.spec_cut_off = cut
     o = seeded_oracle(cfg)
