@@ -29,24 +29,27 @@ def alg_bytes_per_site(s, N, rows_in_sweep=True):
     """Compulsory HBM bytes per site-update of each kernel class of this
     pipeline (s = bytes per real, r = (N/2+1)/N modes per site).
 
+    The stress is stored as 10 components (xx - zz, xy, xz, yy - zz, yz of
+    both RK2 stages): the TT projection removes any multiple of delta_ij.
+
     sweep_a    reads the 4 state arrays; with the k-rows transformed in the
-               sweep writes the 12 half-spectrum rows as N/2 complex each
+               sweep writes the 10 half-spectrum rows as N/2 complex each
                (kz = 0 and N/2, both real, packed in one slot), otherwise the
-               12 real stress arrays
+               10 real stress arrays
     stress_fft j columns: reads the packed rows and writes them back with
-               kz = N/2 split off into a plane of its own (the 12 half-spectra);
+               kz = N/2 split off into a plane of its own (the 10 half-spectra);
                without rows in the sweep, the library k-row R2C and j columns
                in place
-    gw_modes   i columns fused with the mode update: reads the 12 half-spectra
+    gw_modes   i columns fused with the mode update: reads the 10 half-spectra
                and h, l (two TT polarisations, complex), writes h, l
     sweep_b    reads and writes the 4 state arrays
     """
     r = (N // 2 + 1) / N
     return {
-        "sweep_a": 16 * s,   # 12 packed rows carry as many bytes as 12 real arrays
+        "sweep_a": 14 * s,   # 10 packed rows carry as many bytes as 10 real arrays
         "finalize": 0.0,
-        "stress_fft": (12 * s + 24 * s * r) if rows_in_sweep else (12 * s + 24 * s * r + 48 * s * r),
-        "gw_modes": 24 * s * r + 16 * s * r,
+        "stress_fft": (10 * s + 20 * s * r) if rows_in_sweep else (10 * s + 20 * s * r + 40 * s * r),
+        "gw_modes": 20 * s * r + 16 * s * r,
         "sweep_b": 8 * s,
     }
 

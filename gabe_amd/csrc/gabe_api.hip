@@ -191,14 +191,14 @@ int launch_gw_ctx(gabe_ctx *c)
     return GABE_OK;
 }
 
-/* kz = N/2 plane [12][N][N] of the current shat buffer when sweep A packs the
- * rows (after the 12 row blocks), else null */
+/* kz = N/2 plane [GABE_NSC][N][N] of the current shat buffer when sweep A packs
+ * the rows (after the GABE_NSC row blocks), else null */
 void *nyq_plane(gabe_ctx *c)
 {
-    return c->rowfft ? (char *)c->shat + 2 * c->esz * 12 * c->N2 * c->kpad : nullptr;
+    return c->rowfft ? (char *)c->shat + 2 * c->esz * GABE_NSC * c->N2 * c->kpad : nullptr;
 }
 
-/* R2C of the 12 stress arrays into shat [12][N][N][kpad] (kz < N/2+1 valid,
+/* R2C of the GABE_NSC stress arrays into shat [GABE_NSC][N][N][kpad] (kz < N/2+1 valid,
  * or kz < N/2 and the kz = N/2 plane when sweep A packs the rows).
  * with_i = false stops after the j columns (the fused kernel does i). */
 int stress_fft(gabe_ctx *c, bool with_i = true, hipStream_t st = nullptr)
@@ -209,9 +209,9 @@ int stress_fft(gabe_ctx *c, bool with_i = true, hipStream_t st = nullptr)
     const long N = c->N, kp = c->kpad;
     const int cols = c->rowfft ? (int)(N / 2) : (int)(N / 2 + 1);
     void *nyq = nyq_plane(c);
-    ColFftView vj{c->shat, kp, N * kp, 0, cols, 1, 12 * N, nyq};       /* along j */
-    ColFftView vi{c->shat, N * kp, N * N * kp, kp, cols, (int)N, 12};  /* along i */
-    ColFftView vn{nyq, N, N * N, 0, (int)N, 1, 12};                    /* kz = N/2 plane along i */
+    ColFftView vj{c->shat, kp, N * kp, 0, cols, 1, GABE_NSC * N, nyq};       /* along j */
+    ColFftView vi{c->shat, N * kp, N * N * kp, kp, cols, (int)N, GABE_NSC};  /* along i */
+    ColFftView vn{nyq, N, N * N, 0, (int)N, 1, GABE_NSC};                    /* kz = N/2 plane along i */
     if (c->fp32) {
         launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, st);
         if (with_i) launch_colfft<float>(vi, c->N, c->fft_cw, c->twiddle, st);
@@ -402,7 +402,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
         HIPCHK(hipMemset(c->fbuf[s], 0, fbytes));
         HIPCHK(hipMemset(c->vbuf[s], 0, fbytes));
     }
-    const int narr = c->alloc_gw ? 12 : c->nf;
+    const int narr = c->alloc_gw ? GABE_NSC : c->nf;
     c->colfft = c->alloc_gw && gabe_colfft_supported(c->N) && !std::getenv("GABE_LIBRARY_3D_FFT");
     c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
     /* k rows in sweep A: up to GABE_BLOCK sites per row in both forms; 512-site
@@ -411,7 +411,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
                 !std::getenv("GABE_LIBRARY_ROWS");
     /* Row length of shat.  Sweep A's rows are N/2 slots, kz = 0 and N/2 packed
      * into slot 0, and the j pass moves kz = N/2 into a plane of its own after
-     * the 12 row blocks (N^2 (N/2) + N^2 = M slots per component): every
+     * the GABE_NSC row blocks (N^2 (N/2) + N^2 = M slots per component): every
      * column chunk is full and every row a whole number of cache lines.  The
      * library row plan writes N/2+1 values, padded to 16 for the column
      * passes. */
@@ -481,7 +481,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     if (c->alloc_gw) {
         if (c->colfft) {
             if (!c->rowfft)
-                CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12 * (size_t)c->N2, 1,
+                CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, GABE_NSC * (size_t)c->N2, 1,
                               (size_t)c->kpad));
             std::vector<double> tw(2 * c->N);
             for (int e = 0; e < c->N; e++) {
@@ -496,7 +496,7 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
                 HIPCHK(hipMemcpy(c->twiddle, tw.data(), 2 * c->esz * c->N, hipMemcpyHostToDevice));
             }
         } else {
-            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, 12));
+            CHK(make_plan(c, c->fft_stress, rocfft_transform_type_real_forward, !c->fp32, GABE_NSC));
         }
     }
     CHK(make_plan(c, c->fft_spec, rocfft_transform_type_real_forward, !c->fp32, c->nf));

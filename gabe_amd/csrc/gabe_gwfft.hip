@@ -5,17 +5,15 @@
  * After the k-row R2C and the j-column transform, one workgroup takes the
  * N x CW tile of modes (all i, CW adjacent kz) at one j, or, when the kz = N/2
  * values sit in a plane of their own, all i at CW adjacent j of that plane.
- * It transforms the 12
- * stress components along i in LDS, CPR components per round.  After each
- * round, every thread projects the freshly transformed components of its own
- * modes onto the two TT polarisations and accumulates them.  Once all 12 are
- * in, it runs both midpoint-RK2 stages of those modes
+ * It transforms the GABE_NSC stored stress components (five per RK2 stage)
+ * along i in LDS, CPR components per round.  After each round, every thread
+ * projects the freshly transformed components of its own modes onto the two
+ * TT polarisations and accumulates them.  Once all are in, it runs both
+ * midpoint-RK2 stages of those modes
  * (g2evolutionpert.cpp:343-516) and writes h and l.  The fully transformed
  * source never goes back to HBM.
  */
 #include <hip/hip_runtime.h>
-
-#include <cstdlib>
 
 #include "gabe_fft_dev.h"
 #include "gabe_gwfft.h"
@@ -48,25 +46,27 @@ __device__ __forceinline__ int trow2(int a, int b) { return (N2 + 1) * a + b; }
 /* tile elements each thread loads per round */
 constexpr int gw_lpt(int N, int CW, int CPR, int NT) { return (CPR * N * CW + NT - 1) / NT; }
 
-/* One round of an RK2 stage: components 6 ST + C0 .. 6 ST + C0 + CPR - 1 of
- * the source (CPR = 6: the whole stage; CPR = 3: half of it, the projection
- * being linear in the components): fill the CPR N x CW tiles (loaded now, or
- * with PF_IN from pf, loaded by the round before), transform along i (two
- * steps, twiddles from twl), return to natural order, and project the owned
- * modes on e+ and ex into acc[mode][pol].  With PF_OUT the next round's tiles
- * are loaded into pf after the first DFT step, so their latency passes behind
- * the second step, the projection and the barriers between them (loaded
- * before the first step they would not fit the register budget). */
-
-template <typename T, int N1, int N2, int CW, int CPR, int NT, int ST, int C0, bool PF_IN, bool PF_OUT,
+/* One round: source components C0 .. C0 + NCR - 1 of the GABE_NSC stored
+ * ones (stage c / GABE_SC, component c mod GABE_SC of (xx - zz, xy, xz,
+ * yy - zz, yz)): fill the NCR N x CW tiles (loaded now, or with PF_IN from
+ * pf, loaded by the round before), transform along i (two steps, twiddles
+ * from twl), return to natural order, and project the owned modes on e+ and
+ * ex, adding into acc0 or acc1 by the component's stage (the projection is
+ * linear in the components, so rounds may split or straddle stages).  With
+ * NNX > 0 the next round's NNX tiles are loaded into pf after the first DFT
+ * step, so their latency passes behind the second step, the projection and
+ * the barriers between them (loaded before the first step they would not fit
+ * the register budget). */
+template <typename T, int N1, int N2, int CW, int CPR, int NT, int C0, int NCR, int NNX, bool PF_IN,
           typename AfterDft>
-__device__ __forceinline__ void stage_rounds(AfterDft after_dft,
+__device__ __forceinline__ void source_round(AfterDft after_dft,
                                              typename cplx_t<T>::type (&tile)[CPR][N1 * N2 + N1][CW],
                                              const typename cplx_t<T>::type *twl,
                                              const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
                                              const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
                                              const T (&bv)[(N1 * N2 * CW + NT - 1) / NT][3],
-                                             typename cplx_t<T>::type (&acc)[(N1 * N2 * CW + NT - 1) / NT][2],
+                                             typename cplx_t<T>::type (&acc0)[(N1 * N2 * CW + NT - 1) / NT][2],
+                                             typename cplx_t<T>::type (&acc1)[(N1 * N2 * CW + NT - 1) / NT][2],
                                              typename cplx_t<T>::type (&pf)[gw_lpt(N1 * N2, CW, CPR, NT)])
 {
     using C = typename cplx_t<T>::type;
@@ -74,19 +74,18 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
     constexpr int MPT = (N * CW + NT - 1) / NT;
     constexpr int LPT = (CPR * N * CW + NT - 1) / NT;
     constexpr int LB = load_batch(LPT);
-    constexpr int S1T = (CPR * N2 * CW + NT - 1) / NT;
-    constexpr int S2T = (CPR * N1 * CW + NT - 1) / NT;
-    static_assert((CPR == 6 && C0 == 0) || (CPR == 3 && (C0 == 0 || C0 == 3)), "rounds cover a stage");
+    constexpr int S1T = (NCR * N2 * CW + NT - 1) / NT;
+    constexpr int S2T = (NCR * N1 * CW + NT - 1) / NT;
+    static_assert(NCR >= 1 && NCR <= CPR && C0 + NCR <= GABE_NSC && NNX <= CPR, "rounds cover the components");
     static_assert(LPT % LB == 0, "loads must split into whole batches");
     {
-        constexpr int c0 = C0;
         auto prefetch = [&] {
 #pragma unroll
             for (int t = 0; t < LPT; t++) {
                 const int e = threadIdx.x + t * NT;
                 const int r = e / (N * CW), rem = e % (N * CW), i = rem / CW, col = rem % CW;
-                if (e < CPR * N * CW && col < width)
-                    pf[t] = Sh[(long)(6 * ST + c0 + CPR + r) * SM + (long)i * rstride + col];
+                if (e < NNX * N * CW && col < width)
+                    pf[t] = Sh[(long)(C0 + NCR + r) * SM + (long)i * rstride + col];
                 else pf[t].x = pf[t].y = (T)0;
             }
         };
@@ -95,7 +94,7 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
             for (int t = 0; t < LPT; t++) {
                 const int e = threadIdx.x + t * NT;
                 const int r = e / (N * CW), rem = e % (N * CW);
-                if (e < CPR * N * CW) tile[r][trow<N2>(rem / CW)][rem % CW] = pf[t];
+                if (e < NCR * N * CW) tile[r][trow<N2>(rem / CW)][rem % CW] = pf[t];
             }
         } else
 #pragma unroll
@@ -105,23 +104,22 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
             for (int t = 0; t < LB; t++) {
                 const int e = threadIdx.x + (t0 + t) * NT;
                 const int r = e / (N * CW), rem = e % (N * CW), i = rem / CW, col = rem % CW;
-                if (e < CPR * N * CW && col < width)
-                    v[t] = Sh[(long)(6 * ST + c0 + r) * SM + (long)i * rstride + col];
+                if (e < NCR * N * CW && col < width)
+                    v[t] = Sh[(long)(C0 + r) * SM + (long)i * rstride + col];
                 else v[t].x = v[t].y = (T)0;
             }
 #pragma unroll
             for (int t = 0; t < LB; t++) {
                 const int e = threadIdx.x + (t0 + t) * NT;
                 const int r = e / (N * CW), rem = e % (N * CW);
-                if (e < CPR * N * CW) tile[r][trow<N2>(rem / CW)][rem % CW] = v[t];
+                if (e < NCR * N * CW) tile[r][trow<N2>(rem / CW)][rem % CW] = v[t];
             }
         }
         __syncthreads();
-
 #pragma unroll
         for (int t = 0; t < S1T; t++) {
             const int task = threadIdx.x + t * NT;
-            if (task < CPR * N2 * CW) {
+            if (task < NCR * N2 * CW) {
                 const int r = task / (N2 * CW), rem = task % (N2 * CW), col = rem % CW, n2 = rem / CW;
                 C x[N1];
 #pragma unroll
@@ -137,13 +135,13 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
             }
         }
         __syncthreads();
-        if constexpr (PF_OUT) prefetch();
+        if constexpr (NNX > 0) prefetch();
 
         C y[S2T][N2];
 #pragma unroll
         for (int t = 0; t < S2T; t++) {
             const int task = threadIdx.x + t * NT;
-            if (task < CPR * N1 * CW) {
+            if (task < NCR * N1 * CW) {
                 const int r = task / (N1 * CW), rem = task % (N1 * CW), col = rem % CW, k1 = rem / CW;
 #pragma unroll
                 for (int n2 = 0; n2 < N2; n2++) y[t][n2] = tile[r][trow2<N2>(k1, n2)][col];
@@ -154,7 +152,7 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
 #pragma unroll
         for (int t = 0; t < S2T; t++) {
             const int task = threadIdx.x + t * NT;
-            if (task < CPR * N1 * CW) {
+            if (task < NCR * N1 * CW) {
                 const int r = task / (N1 * CW), rem = task % (N1 * CW), col = rem % CW, k1 = rem / CW;
 #pragma unroll
                 for (int k2 = 0; k2 < N2; k2++) tile[r][trow<N2>(k1 + N1 * k2)][col] = y[t][k2];
@@ -172,18 +170,19 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
             if (e < N * CW) {
                 const int i = e / CW, col = e % CW;
 #pragma unroll
-                for (int r = 0; r < CPR; r++) {
-                    const int cc = c0 + r;   /* 11,12,13,22,23,33 */
-                    const int ia = cc < 3 ? 0 : (cc < 5 ? 1 : 2);
+                for (int r = 0; r < NCR; r++) {
+                    const int cc = (C0 + r) % GABE_SC;   /* xx - zz, xy, xz, yy - zz, yz */
+                    const int ia = cc < 3 ? 0 : 1;
                     const int ib = cc == 0 ? 0 : (cc == 1 || cc == 3 ? 1 : 2);
-                    const T ua = ia == 0 ? bu[u][0] : (ia == 1 ? bu[u][1] : bu[u][2]);
-                    const T va = ia == 0 ? bv[u][0] : (ia == 1 ? bv[u][1] : bv[u][2]);
+                    const T ua = ia == 0 ? bu[u][0] : bu[u][1];
+                    const T va = ia == 0 ? bv[u][0] : bv[u][1];
                     const T ub = ib == 0 ? bu[u][0] : (ib == 1 ? bu[u][1] : bu[u][2]);
                     const T vb = ib == 0 ? bv[u][0] : (ib == 1 ? bv[u][1] : bv[u][2]);
                     const T w = ia == ib ? (T)0.5 : (T)1;
                     const T ep = w * (ua * ub - va * vb);
                     const T ex = w * (ua * vb + va * ub);
                     const C S = tile[r][trow<N2>(i)][col];
+                    C(&acc)[MPT][2] = (C0 + r) / GABE_SC == 0 ? acc0 : acc1;
                     acc[u][0].x += ep * S.x;
                     acc[u][0].y += ep * S.y;
                     acc[u][1].x += ex * S.x;
@@ -195,12 +194,40 @@ __device__ __forceinline__ void stage_rounds(AfterDft after_dft,
     }
 }
 
+/* rounds R, R + 1, ... of CPR components until all GABE_NSC are in; the last
+ * round runs last_dft after its transform (the mode loads) */
+template <typename T, int N1, int N2, int CW, int CPR, int NT, int R, typename Nothing, typename LastDft>
+__device__ __forceinline__ void source_rounds(Nothing nothing, LastDft last_dft,
+                                              typename cplx_t<T>::type (&tile)[CPR][N1 * N2 + N1][CW],
+                                              const typename cplx_t<T>::type *twl,
+                                              const typename cplx_t<T>::type *Sh, long SM, long rstride, int width,
+                                              const T (&bu)[(N1 * N2 * CW + NT - 1) / NT][3],
+                                              const T (&bv)[(N1 * N2 * CW + NT - 1) / NT][3],
+                                              typename cplx_t<T>::type (&acc0)[(N1 * N2 * CW + NT - 1) / NT][2],
+                                              typename cplx_t<T>::type (&acc1)[(N1 * N2 * CW + NT - 1) / NT][2],
+                                              typename cplx_t<T>::type (&pf)[gw_lpt(N1 * N2, CW, CPR, NT)])
+{
+    constexpr int C0 = R * CPR;
+    constexpr int NCR = GABE_NSC - C0 < CPR ? GABE_NSC - C0 : CPR;
+    constexpr int LEFT = GABE_NSC - C0 - NCR;
+    constexpr int NNX = LEFT < CPR ? LEFT : CPR;
+    if constexpr (NNX == 0) {
+        source_round<T, N1, N2, CW, CPR, NT, C0, NCR, 0, (R > 0)>(last_dft, tile, twl, Sh, SM, rstride, width, bu, bv,
+                                                                 acc0, acc1, pf);
+    } else {
+        source_round<T, N1, N2, CW, CPR, NT, C0, NCR, NNX, (R > 0)>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv,
+                                                                   acc0, acc1, pf);
+        source_rounds<T, N1, N2, CW, CPR, NT, R + 1>(nothing, last_dft, tile, twl, Sh, SM, rstride, width, bu, bv,
+                                                     acc0, acc1, pf);
+    }
+}
+
 /* waves per SIMD the register allocation must leave room for: the FP32
- * 256^3 tile of 4 columns x 3 components on 256 threads fits 3 without
- * spilling */
+ * 256^3 tiles of 4 columns x 2 or 3 components per round on 256 threads fit
+ * 3 without spilling */
 constexpr int gw_min_waves(int tsize, int N1, int N2, int CW, int CPR, int NT)
 {
-    return tsize == 4 && N1 == 16 && N2 == 16 && CW == 4 && CPR == 3 && NT == 256 ? 3 : 1;
+    return tsize == 4 && N1 == 16 && N2 == 16 && CW == 4 && CPR <= 3 && NT == 256 ? 3 : 1;
 }
 
 template <typename T, int N1, int N2, int CW, int CPR, int NT>
@@ -292,16 +319,8 @@ void k_gw_ifft(GwFftArgs<T> A)
     auto nothing = [] {};
     /* the next round's tiles, carried in registers from one round to the next */
     C pf[gw_lpt(N, CW, CPR, NT)];
-    stage_rounds<T, N1, N2, CW, CPR, NT, 0, 0, false, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0, pf);
-    if constexpr (CPR == 3) {
-        stage_rounds<T, N1, N2, CW, CPR, NT, 0, 3, true, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc0, pf);
-        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0, true, true>(nothing, tile, twl, Sh, SM, rstride, width, bu, bv, acc1, pf);
-        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 3, true, false>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1,
-                                                                 pf);
-    } else {
-        stage_rounds<T, N1, N2, CW, CPR, NT, 1, 0, true, false>(load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc1,
-                                                                 pf);
-    }
+    source_rounds<T, N1, N2, CW, CPR, NT, 0>(nothing, load_modes, tile, twl, Sh, SM, rstride, width, bu, bv, acc0, acc1,
+                                             pf);
 
     /* both midpoint-RK2 stages of the owned modes (derivs, g2evolutionpert.cpp:486-501) */
     const StepScalars *ss = A.ss;
@@ -351,14 +370,10 @@ void launch_cfg(const GwFftArgs<T> &a, hipStream_t s)
 
 bool gabe_gwfft_supported(int N, bool fp32) { return N >= 16 && N <= (fp32 ? 512 : 256) && (N & (N - 1)) == 0; }
 
-/* Tile variants (kz columns per workgroup CW, components per round CPR,
- * threads NT) at the sizes where more than one was worth keeping; variant 1
- * is the built-in default, the rest are the alternatives the autotuner
- * times.  At 256^3 FP32 the default (4 columns, 3 components per round, 256
- * threads: 3 workgroups per CU) measured fastest once the mode loads moved
- * into the last round; at 512^3 the 512-thread tile shares the CUs better
- * with the sweeps (a 7 % shorter pipelined step than 2 x 6 on 256 threads at
- * the same kernel time). */
+/* Tile variants (kz columns per workgroup CW, source components per round
+ * CPR, threads NT) at the sizes where more than one was worth keeping;
+ * variant 1 is the built-in default, the rest are the alternatives the
+ * autotuner times. */
 int gabe_gwfft_tiles(int N, bool fp32)
 {
     if (!gabe_gwfft_supported(N, fp32)) return 0;
@@ -370,26 +385,28 @@ template <>
 void launch_gw_ifft<float>(const GwFftArgs<float> &a, hipStream_t s)
 {
     switch (a.N) {
-    case 16: launch_cfg<float, 4, 4, 8, 6, 256>(a, s); break;
-    case 32: launch_cfg<float, 8, 4, 8, 6, 256>(a, s); break;
-    case 64: launch_cfg<float, 8, 8, 8, 6, 512>(a, s); break;
-    case 128: launch_cfg<float, 16, 8, 8, 6, 1024>(a, s); break;
+    case 16: launch_cfg<float, 4, 4, 8, 5, 256>(a, s); break;
+    case 32: launch_cfg<float, 8, 4, 8, 5, 256>(a, s); break;
+    case 64: launch_cfg<float, 8, 8, 8, 5, 512>(a, s); break;
+    case 128: launch_cfg<float, 16, 8, 8, 5, 1024>(a, s); break;
     case 256:
+        /* default: 4 columns, 2 of the 10 components per round, 256 threads
+         * (3 workgroups per CU); 3 components per round measures the same */
         switch (a.tile) {
-        case 2: launch_cfg<float, 16, 16, 4, 6, 512>(a, s); break;
-        case 3: launch_cfg<float, 16, 16, 4, 6, 256>(a, s); break;
-        case 4: launch_cfg<float, 16, 16, 8, 6, 512>(a, s); break;
-        case 5: launch_cfg<float, 16, 16, 8, 3, 512>(a, s); break;
-        case 6: launch_cfg<float, 16, 16, 4, 3, 512>(a, s); break;
-        case 7: launch_cfg<float, 16, 16, 8, 6, 1024>(a, s); break;
-        default: launch_cfg<float, 16, 16, 4, 3, 256>(a, s); break;
+        case 2: launch_cfg<float, 16, 16, 4, 3, 256>(a, s); break;
+        case 3: launch_cfg<float, 16, 16, 4, 5, 512>(a, s); break;
+        case 4: launch_cfg<float, 16, 16, 4, 5, 256>(a, s); break;
+        case 5: launch_cfg<float, 16, 16, 8, 5, 512>(a, s); break;
+        case 6: launch_cfg<float, 16, 16, 8, 2, 512>(a, s); break;
+        case 7: launch_cfg<float, 16, 16, 4, 10, 1024>(a, s); break;
+        default: launch_cfg<float, 16, 16, 4, 2, 256>(a, s); break;
         }
         break;
     case 512:
         switch (a.tile) {
-        case 2: launch_cfg<float, 32, 16, 2, 6, 256>(a, s); break;
-        case 3: launch_cfg<float, 32, 16, 2, 6, 512>(a, s); break;
-        case 4: launch_cfg<float, 32, 16, 2, 3, 256>(a, s); break;
+        case 2: launch_cfg<float, 32, 16, 4, 2, 512>(a, s); break;
+        case 3: launch_cfg<float, 32, 16, 2, 5, 256>(a, s); break;
+        case 4: launch_cfg<float, 32, 16, 2, 5, 512>(a, s); break;
         default: launch_cfg<float, 32, 16, 4, 3, 512>(a, s); break;
         }
         break;
@@ -403,15 +420,15 @@ template <>
 void launch_gw_ifft<double>(const GwFftArgs<double> &a, hipStream_t s)
 {
     switch (a.N) {
-    case 16: launch_cfg<double, 4, 4, 8, 6, 256>(a, s); break;
-    case 32: launch_cfg<double, 8, 4, 8, 6, 256>(a, s); break;
-    case 64: launch_cfg<double, 8, 8, 4, 6, 256>(a, s); break;
-    case 128: launch_cfg<double, 16, 8, 4, 6, 512>(a, s); break;
+    case 16: launch_cfg<double, 4, 4, 8, 5, 256>(a, s); break;
+    case 32: launch_cfg<double, 8, 4, 8, 5, 256>(a, s); break;
+    case 64: launch_cfg<double, 8, 8, 4, 5, 256>(a, s); break;
+    case 128: launch_cfg<double, 16, 8, 4, 5, 512>(a, s); break;
     case 256:
         switch (a.tile) {
-        case 2: launch_cfg<double, 16, 16, 2, 3, 256>(a, s); break;
-        case 3: launch_cfg<double, 16, 16, 4, 3, 256>(a, s); break;
-        default: launch_cfg<double, 16, 16, 2, 6, 256>(a, s); break;
+        case 2: launch_cfg<double, 16, 16, 2, 2, 256>(a, s); break;
+        case 3: launch_cfg<double, 16, 16, 2, 3, 256>(a, s); break;
+        default: launch_cfg<double, 16, 16, 2, 5, 256>(a, s); break;
         }
         break;
     default: break;

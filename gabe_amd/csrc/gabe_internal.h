@@ -14,6 +14,14 @@
 #define GABE_BLOCK 256
 #define GABE_NXCD 8         /* accelerator complex dies of an MI355X, each with its own L2 */
 #define GABE_HIST_COLS 8
+/* Stress components transformed per RK2 stage.  The TT projection
+ * Lambda_ij,lm = P_il P_jm - P_ij P_lm / 2 annihilates any multiple of
+ * delta_lm (tr P = 2 for k != 0), so the source S of g2evolutionpert.cpp:48-132
+ * and S - S_zz delta give the same T = Lambda S: the five components
+ * (xx - zz, xy, xz, yy - zz, yz) carry it, the zz one is zero.  Stage 0
+ * (S(f0)) is components 0-4, stage 1 (S(f1)) 5-9. */
+#define GABE_SC 5
+#define GABE_NSC (2 * GABE_SC)
 
 /* Persistent device scalars: the reference's a[2], adot[2], ed*[2]
  * (g2header.h:59-64) plus, per field, the FP64 background (mean field pbar and
@@ -56,9 +64,9 @@ struct SweepArgs {
     const T *v[GABE_MAXF];  /* time derivative, input state */
     T *fo[GABE_MAXF];       /* sweep B output state */
     T *vo[GABE_MAXF];
-    T *stress;              /* [12][N^3]: S(f0) comps 0-5, S(f1) comps 6-11 */
-    T *shat;                /* [12][N][N][kpad = N/2] complex: row transforms written by the fused sweep A,
-                             * slot 0 = S(kz 0) + i S(kz N/2) */
+    T *stress;              /* [GABE_NSC][N^3]: S(f0) comps 0-4, S(f1) comps 5-9 */
+    T *shat;                /* [GABE_NSC][N][N][kpad = N/2] complex: row transforms written by the fused
+                             * sweep A, slot 0 = S(kz 0) + i S(kz N/2) */
     const T *tw;            /* exp(-2 pi i e / N), e < N, complex */
     long kpad;
     int rowfft;             /* N when sweep A transforms its k-rows itself, else 0 */
@@ -84,7 +92,7 @@ struct FinArgs {
 
 template <typename T>
 struct GwArgs {
-    const T *shat;      /* [12][N][N][kpad] complex: S(f0) transforms 0-5, S(f1) 6-11 */
+    const T *shat;      /* [GABE_NSC][N][N][kpad] complex: S(f0) transforms 0-4, S(f1) 5-9 */
     T *h, *l;           /* [M][2] complex: (+, x) polarisation amplitudes per mode */
     const StepScalars *ss;
     int N;

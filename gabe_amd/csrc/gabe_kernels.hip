@@ -2,7 +2,7 @@
  * gabe_kernels.hip -- CDNA4 (gfx950) kernels of the lattice evolution.
  *
  * One step (expansion_type 1, step(), g2functions.cpp:234-270) is
- *   sweep A  -> finalize -> [R2C of the 12 stress arrays -> GW mode kernel]
+ *   sweep A  -> finalize -> [R2C of the GABE_NSC stress arrays -> GW mode kernel]
  *                        -> sweep B
  * Sweep A reads the step-start state once (7-point halo) and produces every
  * quantity that needs a global reduction: the energies of the step-start state
@@ -80,13 +80,14 @@ __device__ void block_store_partials(double (&acc)[NS], double *partials, int nb
 }
 
 /* ------------------------------------------------------------------ sweeps */
-/* Row R2C of the 12 stress components of the row_block(N) / N k-rows one sweep
- * workgroup holds, for npl (1 or 2) consecutive planes from plane i, written
- * into the padded half-spectrum rows of shat (the library row transform it
- * replaces).  Each stage's components arrive in LDS as three complex rows,
- * z = S_c + i S_(c+1) for the pairs (xx,xy), (xz,yy), (yz,zz); one complex
- * transform of length N per pair (two-step N = N1 N2, in-register DFTs) and
- * the split
+/* Row R2C of the GABE_NSC stress components of the row_block(N) / N k-rows
+ * one sweep workgroup holds, for npl (1 or 2) consecutive planes from plane
+ * i, written into the half-spectrum rows of shat (the library row transform
+ * it replaces).  The components arrive in LDS as GABE_NSC / 2 complex rows,
+ * z = S_c + i S_(c+1) for c = 0, 2, ..., 8 (the pairs (xx,xy), (xz,yy) of
+ * stage 0, (yz of stage 0, xx of stage 1), (xy,xz), (yy,yz) of stage 1); one
+ * complex transform of length N per pair (two-step N = N1 N2, in-register
+ * DFTs) and the split
  *   S_c^(m) = (Z(m) + conj Z(N-m)) / 2,  S_(c+1)^(m) = (Z(m) - conj Z(N-m)) / 2i
  * give both real rows' transforms, m = 0..N/2.  Rows are stored N/2 slots
  * long: S(0) and S(N/2), both real, share slot 0 as S(0) + i S(N/2) (the
@@ -105,9 +106,12 @@ __host__ __device__ constexpr int row_block(int N) { return N > GABE_BLOCK ? N :
  * same LDS bytes) */
 template <typename T>
 __host__ __device__ constexpr int row_planes(int N) { return sizeof(T) == 4 && N <= GABE_BLOCK ? 2 : 1; }
-/* complex slots of one buffer: row_planes x 6 pairs x rows per workgroup */
+/* complex slots of one buffer: row_planes x GABE_NSC / 2 pairs x rows per workgroup */
 template <typename T>
-__host__ __device__ constexpr int rbuf_size(int N) { return row_planes<T>(N) * 6 * (row_block(N) / N) * rseq_stride(N); }
+__host__ __device__ constexpr int rbuf_size(int N)
+{
+    return row_planes<T>(N) * (GABE_NSC / 2) * (row_block(N) / N) * rseq_stride(N);
+}
 
 template <typename T, int N>
 __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, typename gabe_fft::cplx_t<T>::type *zo,
@@ -119,7 +123,8 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     constexpr int N1 = N == 16 ? 4 : (N <= 64 ? 8 : 16);
     constexpr int N2 = N / N1;
     constexpr int BLK = row_block(N);
-    constexpr int R = BLK / N, SPP = 6 * R, H = N / 2;   /* sequences per plane, slots per row */
+    constexpr int NP = GABE_NSC / 2;                     /* complex sequences (pairs) per row */
+    constexpr int R = BLK / N, SPP = NP * R, H = N / 2;  /* sequences per plane, slots per row */
     constexpr int SS = rseq_stride(N);
     static_assert((N1 - 1) * (N2 - 1) < N, "twiddle exponents index one table of N");
     const int nseq = SPP * npl;
@@ -165,19 +170,20 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
     if constexpr (R == 1 && BLK == 2 * H) {
         /* one k-row per workgroup (N = 256, 512): thread t keeps slot
          * m = t mod H and takes sequences s0, s0 + 2, ... (s0 = t / H), i.e.
-         * pairs s0, s0 + 2, s0 + 4 of plane 0, then of plane 1.  The slot's
-         * LDS offsets and row address are computed once. */
+         * pairs s0, s0 + 2, s0 + 4 (below NP) of plane 0, then of plane 1.
+         * The slot's LDS offsets and row address are computed once. */
         const int m = threadIdx.x % H, s0 = threadIdx.x / H;
         const bool slot0 = m == 0;   /* slot 0 packs S(0) + i S(N/2) */
-        const C *o = zo + s0 * SS;   /* sequence s0 + 2 q of plane pl at o + (6 pl + 2 q) SS */
+        const C *o = zo + s0 * SS;   /* sequence s0 + 2 q of plane pl at o + (NP pl + 2 q) SS */
         const int iz = rpad(m), iw = rpad(slot0 ? N / 2 : N - m);
         C *dst = out + (long)(2 * s0) * comp + ((long)i * N + j0) * A.kpad + m;
         const long plane_stride = (long)N * A.kpad;
         /* planes one at a time: unrolling both keeps too many values live */
 #pragma unroll 1
-        for (int pl = 0; pl < npl; pl++, o += 6 * SS, dst += plane_stride) {
+        for (int pl = 0; pl < npl; pl++, o += NP * SS, dst += plane_stride) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) {   /* pair s0 + 2 q of plane pl */
+            for (int q = 0; q < (NP + 1) / 2; q++) {   /* pair s0 + 2 q of plane pl */
+                if (s0 + 2 * q >= NP) break;
                 const C Z = o[2 * q * SS + iz], W = o[2 * q * SS + iw];
                 C s0v, s1v;
                 s0v.x = slot0 ? Z.x : h * (Z.x + W.x);
@@ -505,8 +511,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                     }
                 }
                 if (MODE == SW_A && GW) {
-                    /* traceless gradient stress (g2evolutionpert.cpp:48-132):
-                     * S_ab = sum_f D_a f D_b f / dx^2 - delta_ab/3 sum_f |D f|^2 / dx^2 */
+                    /* gradient stress of both stages (g2evolutionpert.cpp:48-132),
+                     * S_ab = sum_f D_a f D_b f / dx^2 up to a multiple of delta_ab,
+                     * as the GABE_SC components (xx - zz, xy, xz, yy - zz, yz): the
+                     * TT projection removes the rest (the reference's trace part
+                     * and the zz component alike) */
+                    T sv[GABE_NSC];
 #pragma unroll
                     for (int st = 0; st < 2; st++) {
                         T sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
@@ -516,29 +526,31 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                             sxx += g[0] * g[0]; syy += g[1] * g[1]; szz += g[2] * g[2];
                             sxy += g[0] * g[1]; sxz += g[0] * g[2]; syz += g[1] * g[2];
                         }
-                        const T q3 = (sxx + syy + szz) * (T)(1. / 3.);
-                        if constexpr (ROWN > 0) {
-                            /* sequence (plane, pair, row) = ((i - i0) mod row_planes, 3 st + q, t / N) */
-                            using C = typename gabe_fft::cplx_t<T>::type;
-                            constexpr int RR = BLK / ROWN, SS = rseq_stride(ROWN), PPC = row_planes<T>(ROWN);
-                            const int t = threadIdx.x, r = t / ROWN, kk = t - r * ROWN;
-                            C *z = zl + ((((i - i0) % PPC) * 6 + 3 * st) * RR + r) * SS + rpad(kk);
-                            C z0, z1, z2;
-                            z0.x = (sxx - q3) * idx2; z0.y = sxy * idx2;
-                            z1.x = sxz * idx2;        z1.y = (syy - q3) * idx2;
-                            z2.x = syz * idx2;        z2.y = (szz - q3) * idx2;
-                            z[0] = z0;
-                            z[RR * SS] = z1;
-                            z[2 * RR * SS] = z2;
-                        } else {
-                            T *S = A.stress + (long)(6 * st) * A.N3 + base + pp;
-                            S[0 * A.N3] = (sxx - q3) * idx2;
-                            S[1 * A.N3] = sxy * idx2;
-                            S[2 * A.N3] = sxz * idx2;
-                            S[3 * A.N3] = (syy - q3) * idx2;
-                            S[4 * A.N3] = syz * idx2;
-                            S[5 * A.N3] = (szz - q3) * idx2;
+                        sv[GABE_SC * st + 0] = (sxx - szz) * idx2;
+                        sv[GABE_SC * st + 1] = sxy * idx2;
+                        sv[GABE_SC * st + 2] = sxz * idx2;
+                        sv[GABE_SC * st + 3] = (syy - szz) * idx2;
+                        sv[GABE_SC * st + 4] = syz * idx2;
+                    }
+                    if constexpr (ROWN > 0) {
+                        /* complex sequence q of a plane carries components 2q (real
+                         * part) and 2q + 1 (imaginary part); sequence (plane, q, row t / N)
+                         * at ((plane mod row_planes) GABE_NSC/2 + q) RR + t / N */
+                        using C = typename gabe_fft::cplx_t<T>::type;
+                        constexpr int RR = BLK / ROWN, SS = rseq_stride(ROWN), PPC = row_planes<T>(ROWN);
+                        const int t = threadIdx.x, r = t / ROWN, kk = t - r * ROWN;
+                        C *z = zl + ((((i - i0) % PPC) * (GABE_NSC / 2)) * RR + r) * SS + rpad(kk);
+#pragma unroll
+                        for (int q = 0; q < GABE_NSC / 2; q++) {
+                            C zq;
+                            zq.x = sv[2 * q];
+                            zq.y = sv[2 * q + 1];
+                            z[q * RR * SS] = zq;
                         }
+                    } else {
+                        T *S = A.stress + base + pp;
+#pragma unroll
+                        for (int c = 0; c < GABE_NSC; c++) S[(long)c * A.N3] = sv[c];
                     }
                 }
             }
@@ -859,11 +871,11 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw(GwArgs<T> G)
     C *hh = reinterpret_cast<C *>(G.h);
     C *ll = reinterpret_cast<C *>(G.l);
     const long SM = (long)N * N * G.kpad, ms = r * G.kpad + pz;
-    C S0[6], S1[6];
+    C S0[GABE_SC], S1[GABE_SC];
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
+    for (int c = 0; c < GABE_SC; c++) {
         S0[c] = Sh[(long)c * SM + ms];
-        S1[c] = Sh[(long)(6 + c) * SM + ms];
+        S1[c] = Sh[(long)(GABE_SC + c) * SM + ms];
     }
     C h0[2] = {hh[2 * m], hh[2 * m + 1]}, l0[2] = {ll[2 * m], ll[2 * m + 1]};
     C T0[2], T1[2];
@@ -875,9 +887,10 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw(GwArgs<T> G)
 #pragma unroll
         for (int q = 0; q < 2; q++)
 #pragma unroll
-            for (int c = 0; c < 6; c++) {
-                /* (e : S)/2 dx^3 with off-diagonal components counted twice */
-                const T w = (T)((c == 0 || c == 3 || c == 5 ? 0.5 : 1.0) * G.dx3 * e[q][c]);
+            for (int c = 0; c < GABE_SC; c++) {
+                /* (e : S)/2 dx^3 with off-diagonal components counted twice; the
+                 * zz component of the stored source is zero */
+                const T w = (T)((c == 0 || c == 3 ? 0.5 : 1.0) * G.dx3 * e[q][c]);
                 T0[q].x += w * S0[c].x; T0[q].y += w * S0[c].y;
                 T1[q].x += w * S1[c].x; T1[q].y += w * S1[c].y;
             }
@@ -957,9 +970,10 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_gw_unpack(const T *pol, double *
     }
 }
 
-/* T_gw[s] in the reference layout [12][M] (diagnostic, not on the step path).
- * With nyq set, shat rows hold kz < N/2 and the kz = N/2 plane is
- * nyq [12][N][N]. */
+/* T_gw[s] in the reference layout [12][M] (diagnostic, not on the step path)
+ * from the stored source (xx - zz, xy, xz, yy - zz, yz), whose projection is
+ * the reference's.  With nyq set, shat rows hold kz < N/2 and the kz = N/2
+ * plane is nyq [GABE_NSC][N][N]. */
 template <typename T>
 __global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, const T *nyq, int stage, int N, long M, long kpad,
                                                          double dx3, double *out)
@@ -974,13 +988,14 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_tt_out(const T *shat, const T *n
     const int px = i <= N / 2 ? i : i - N, py = j <= N / 2 ? j : j - N, pz = kz;
     const int p2 = px * px + py * py + pz * pz;
     const long SM = (long)N * N * kpad;
-    const C *Sh = reinterpret_cast<const C *>(shat) + (long)(6 * stage) * SM;
+    const C *Sh = reinterpret_cast<const C *>(shat) + (long)(GABE_SC * stage) * SM;
     const bool plane = nyq != nullptr && kz == N / 2;
     const long NN = (long)N * N;
-    const C *Sn = reinterpret_cast<const C *>(nyq) + (long)(6 * stage) * NN;
+    const C *Sn = reinterpret_cast<const C *>(nyq) + (long)(GABE_SC * stage) * NN;
     C S[6], Tt[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) S[c] = plane ? Sn[(long)c * NN + r] : Sh[(long)c * SM + r * kpad + kz];
+    for (int c = 0; c < GABE_SC; c++) S[c] = plane ? Sn[(long)c * NN + r] : Sh[(long)c * SM + r * kpad + kz];
+    S[5].x = S[5].y = (T)0;   /* zz */
     if (p2 == 0) {
 #pragma unroll
         for (int c = 0; c < 6; c++) Tt[c].x = Tt[c].y = 0;

@@ -149,7 +149,8 @@ def test_zero_closure_is_the_plain_step_and_fallback_switches_it_off():
 def stored_arrays(lat):
     """The FP32 lattice's field and velocity arrays as stored on the device
     (fluctuations about the FP64 means): the tail of its checkpoint blob when
-    the lattice carries no tensor modes."""
+    the lattice carries no tensor modes and no closure (an attached closure's
+    state follows the arrays)."""
     n = 2 * lat.N ** 3
     tail = np.frombuffer(lat.checkpoint()[-8 * n:].tobytes(), dtype=np.float32)
     shp = (2, lat.N, lat.N, lat.N)
@@ -183,6 +184,7 @@ def test_step_correction_is_the_network_kick(N):
     for a, b in zip(A.download_state()[0], B.download_state()[0]):
         assert np.array_equal(a, b)
     fB, vB = stored_arrays(B)
+    cm.detach(A)
     _, vA = stored_arrays(A)
     inv = [np.float32(1.0 / s) for s in s_in]
     x = np.stack([fB[0] * inv[0], fB[1] * inv[1], vB[0] * inv[2], vB[1] * inv[3]], axis=-1)
