@@ -323,10 +323,13 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
         }
         __builtin_amdgcn_sched_barrier(0);
         if (valid) {
+            double V0, V1, dV0[2], dV1[2];
+            gabe_V_dV(A.model, full0[0], full0[1], V0, dV0[0], dV0[1]);
+            gabe_V_dV(A.model, full1[0], full1[1], V1, dV1[0], dV1[1]);
 #pragma unroll
             for (int f = 0; f < NF; f++) {
-                const double Vp0 = gabe_dVdf(A.model, f, full0[0], full0[1]);
-                const double Vp1 = gabe_dVdf(A.model, f, full1[0], full1[1]);
+                const double Vp0 = dV0[f];
+                const double Vp1 = dV1[f];
                 const double dv0 = (double)bv[QC][f];
                 /* F(0) and F(1) of ddfield (g2functions.cpp:193-196) minus the
                  * background force */
@@ -476,11 +479,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
         }
 
         if (valid) {
+            /* V and V' at the step-start and midpoint fields (one evaluation
+             * each gives V and both derivatives) */
+            double V0, V1 = 0., dV0[2], dV1[2] = {0., 0.};
+            gabe_V_dV(A.model, full0[0], full0[1], V0, dV0[0], dV0[1]);
+            if (MODE != SW_ENERGY) gabe_V_dV(A.model, full1[0], full1[1], V1, dV1[0], dV1[1]);
             if (MODE == SW_B) {
 #pragma unroll
                 for (int f = 0; f < NF; f++) {
-                    const double Vp0 = gabe_dVdf(A.model, f, full0[0], full0[1]);
-                    const double Vp1 = gabe_dVdf(A.model, f, full1[0], full1[1]);
+                    const double Vp0 = dV0[f];
+                    const double Vp1 = dV1[f];
                     const double dv0 = (double)vc[f];
                     /* F(0) and F(1) of ddfield (g2functions.cpp:193-196) minus the
                      * background force (zero in the FP64 form) */
@@ -491,8 +499,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                     A.vo[f][base + pp] = (T)(dv0 + dt * dF1);
                 }
             } else {
-                acc[0] += gabe_V(A.model, full0[0], full0[1]);
-                if (MODE == SW_A) acc[1] += gabe_V(A.model, full1[0], full1[1]);
+                acc[0] += V0;
+                if (MODE == SW_A) acc[1] += V1;
 #pragma unroll
                 for (int f = 0; f < NF; f++) {
                     const double vel0 = vb0[f] + (double)vc[f];
@@ -500,8 +508,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
                     acc[b + 0] += vel0 * vel0;
                     acc[b + 3] += (double)fc[f] * (double)lap0[f];
                     if (MODE == SW_A) {
-                        const double Vp0 = gabe_dVdf(A.model, f, full0[0], full0[1]);
-                        const double Vp1 = gabe_dVdf(A.model, f, full1[0], full1[1]);
+                        const double Vp0 = dV0[f];
+                        const double Vp1 = dV1[f];
                         const double X = (double)lap0[f] * ia02 - Vp0;
                         acc[b + 1] += vel0 * X;
                         acc[b + 2] += X * X;
