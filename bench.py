@@ -139,16 +139,14 @@ def precision_report(f32, f64):
 
 def run_ensemble(steps, warmup):
     """Aggregate site-updates/s of the batched ensemble against the same 64
-    lattices run one after another, each to completion, on their own contexts."""
-    from gabe_amd import host
-    from gabe_amd.lattice import GABE_FP32, Ensemble, Lattice, autotune_sweep
-    from gabe_amd import configs
+    lattices run one after another, each to completion, on their own contexts;
+    per-phase HIP-event times of the batched launches and the roofline of the
+    longest (fields only: sweep A reads the 4 state arrays, sweep B reads and
+    writes them)."""
+    from gabe_amd import configs, host
+    from gabe_amd.lattice import GABE_FP32, Ensemble, Lattice
     cfgs = configs.ensemble_configs()
     sites = sum(c.N ** 3 for c in cfgs)
-    tuned = {}
-    for c in cfgs:
-        if c.N not in tuned:
-            tuned[c.N] = autotune_sweep(precision=GABE_FP32, **c.lattice_kwargs())[0]
     ens = Ensemble([dict(precision=GABE_FP32, **c.lattice_kwargs()) for c in cfgs])
     for c, m in zip(cfgs, ens.members):
         host.initialize_member(m, c)
@@ -158,12 +156,15 @@ def run_ensemble(steps, warmup):
     ens.step(steps)
     ens.sync()
     t_ens = time.perf_counter() - t0
+    ktimes = ens.kernel_times(20)
+    ens.sync()
     ppb = [m.query().planes_per_block for m in ens.members]
+    tiling = {str(c.N): p for c, p in zip(cfgs, ppb)}
     ens.close()
 
     singles = []
-    for c, p in zip(cfgs, ppb):
-        s = Lattice(precision=GABE_FP32, planes_per_block=p, **c.lattice_kwargs())
+    for c in cfgs:
+        s = Lattice(precision=GABE_FP32, **c.lattice_kwargs())
         host.initialize(s, c)
         s.step(warmup)
         s.sync()
@@ -175,14 +176,23 @@ def run_ensemble(steps, warmup):
     t_seq = time.perf_counter() - t0
     for s in singles:
         s.close()
+    rate = sites * steps / t_ens
+    alg = {"sweep_a": 16.0, "sweep_b": 32.0}   # bytes per site of the FP32 state arrays
+    dom = max(alg, key=lambda k: ktimes[k])
+    achieved = alg[dom] * sites / (ktimes[dom] * 1e-3) / 1e9
     return {
-        "value": sites * steps / t_ens,
+        "value": rate,
         "unit": "site-updates/s",
         "sequential_value": sites * steps / t_seq,
         "speedup_vs_sequential": t_seq / t_ens,
         "ms_per_step": 1e3 * t_ens / steps,
         "sequential_ms_per_step": 1e3 * t_seq / steps,
-        "autotuned_planes_per_block": {str(n): p for n, p in tuned.items()},
+        "kernels_ms": ktimes,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg[dom] * sites},
+        "fields_frac": 48.0 * rate / (HBM_PEAK_GBS * 1e9),
+        "planes_per_block": tiling,
         "lattices": len(cfgs),
         "sites": sites,
         "steps": steps,

@@ -242,13 +242,87 @@ def evaluate(cl, setup, seed, coarse_steps, threshold_factor=10.0, gws=True, log
     return out
 
 
+def validation_case(setup, seed, coarse_steps):
+    """A held-out run for choosing among trained networks: the restricted
+    coarse start state (a checkpoint blob) and the fine run's spectra after
+    coarse_steps * ratio steps, fields only."""
+    fine = host.initialized_lattice(_with_seed(setup.fine_cfg, seed, gws=0), precision=GABE_FP32)
+    coarse = Lattice(precision=GABE_FP32, **_with_seed(setup.coarse_cfg, seed, gws=0).lattice_kwargs())
+    restrict(fine, coarse)
+    start = coarse.checkpoint()
+    coarse.close()
+    fine.step(coarse_steps * setup.ratio)
+    spf = fine.specOut()
+    fine.close()
+    return start, spf
+
+
+def validation_error(cl, setup, case, coarse_steps):
+    """Worst and mean per-bin field-spectrum error of a closure run from the
+    case's start state (no fallback), against the case's fine spectra."""
+    start, spf = case
+    lat = Lattice(precision=GABE_FP32, **setup.coarse_cfg.lattice_kwargs())
+    lat.restore(start)
+    attach(lat, cl, 1e300)
+    lat.step(coarse_steps)
+    err = spectral_error(lat.specOut()[[0, 2]], spf[[0, 2]], setup.coarse_cfg.N // 2)
+    lat.close()
+    return float(err.max()), float(err.mean())
+
+
+def train_validated(cl, setup, schedule, case, coarse_steps, check_every, log=None):
+    """Adam phases (lr, iterations); every check_every iterations the network
+    is run on the validation case, and the parameters with the smallest worst
+    bin are kept (the training loss weights the short wavelengths, where the
+    coarse Laplacian's error is largest, and is not the rollout's error).
+    Returns the loss per iteration and the validation history."""
+    losses, hist = [], []
+    best = (float("inf"), None)
+    for n, (lr, iters) in enumerate(schedule):
+        done = 0
+        while done < iters:
+            k = min(check_every, iters - done)
+            losses += list(cl.train(k, lr=lr, seed=1000 * n + done))
+            done += k
+            worst, mean = validation_error(cl, setup, case, coarse_steps)
+            hist.append({"iteration": len(losses), "lr": lr, "worst": worst, "mean": mean})
+            if log:
+                log(f"iteration {len(losses)} (lr {lr:g}): validation worst bin {worst:.4f}, mean {mean:.4f}")
+            if worst < best[0]:
+                best = (worst, cl.params().copy())
+    if best[1] is not None:
+        si, so = cl.scales()
+        cl.set_params(best[1])
+        cl.set_scales(si, so)
+    return losses, hist
+
+
+def summarize(per_seed, gws=True):
+    """Worst and mean per-bin errors over all held-out seeds, both fields."""
+    out = {}
+    for name in ("plain", "closure"):
+        errs = np.array([r[f"{name}_field_err_bins"] for r in per_seed.values()])   # [seed][field][bin]
+        out[f"{name}_field_err_max"] = float(errs.max())
+        out[f"{name}_field_err_mean"] = float(errs.mean())
+        out[f"{name}_bins_within_5pct"] = float((errs <= 0.05).mean())
+        worst = np.unravel_index(np.argmax(errs), errs.shape)
+        out[f"{name}_worst_at"] = {"seed": list(per_seed)[worst[0]], "field": int(worst[1]), "bin": int(worst[2]) + 1}
+        if gws:
+            out[f"{name}_gw_err_mean"] = float(np.mean([r[f"{name}_gw_err_mean"] for r in per_seed.values()]))
+            out[f"{name}_gw_err_max"] = float(np.max([r[f"{name}_gw_err_max"] for r in per_seed.values()]))
+    out["closure_cost_vs_plain"] = float(np.mean([r["closure_cost_vs_plain"] for r in per_seed.values()]))
+    out["speedup_vs_fine"] = float(np.mean([r["speedup_vs_fine"] for r in per_seed.values()]))
+    return out
+
+
 def main(argv=None):
     """Train a closure on fine runs and evaluate it against a held-out fine run.
 
     Defaults: 128^3 coarse lattice, 512^3 fine lattice (ratio 4, dt = dx/20
     on both), lambda phi^4 + g^2 phi^2 chi^2 model of the benchmark, training
-    seeds 1 and 2, held-out seed 3, 1000 coarse steps with tensor modes in the
-    evaluated runs.  Prints one JSON line and writes the trained network."""
+    seeds 1 and 2, held-out seeds 3, 4 and 5, 1000 coarse steps with tensor
+    modes in the evaluated runs.  Prints one JSON line (per seed and a summary
+    over the seeds) and writes the trained network."""
     import argparse
     import json
     import sys
@@ -260,9 +334,12 @@ def main(argv=None):
     ap.add_argument("--train-seeds", type=int, nargs="+", default=[1, 2])
     ap.add_argument("--samples", type=int, default=100, help="training pairs per seed")
     ap.add_argument("--stride", type=int, default=10, help="coarse steps between pairs")
-    ap.add_argument("--iters", type=int, default=2000)
-    ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--eval-seed", type=int, default=3)
+    ap.add_argument("--schedule", default="1e-3:2000",
+                    help="Adam phases as lr:iterations, comma separated (the moments carry over)")
+    ap.add_argument("--val-seed", type=int, default=0,
+                    help="validation seed: keep the parameters whose closure run matches it best (0: last)")
+    ap.add_argument("--check-every", type=int, default=250, help="iterations between validation runs")
+    ap.add_argument("--eval-seeds", type=int, nargs="+", default=[3, 4, 5], help="held-out seeds")
     ap.add_argument("--steps", type=int, default=1000, help="coarse steps of the evaluated runs")
     ap.add_argument("--threshold-factor", type=float, default=10.0,
                     help="fallback once the Friedmann residual exceeds this times the plain run's")
@@ -288,14 +365,33 @@ def main(argv=None):
         l0 = float(np.mean([cl.loss(s) for s in range(cl.num_samples())]))
         log(f"{cl.num_samples()} samples, initial loss {l0:.4f}")
         t0 = time.perf_counter()
-        losses = cl.train(args.iters, lr=args.lr, seed=1)
+        schedule = [(float(p.split(":")[0]), int(p.split(":")[1])) for p in args.schedule.split(",")]
+        if args.val_seed:
+            case = validation_case(setup, args.val_seed, args.steps)
+            losses, vhist = train_validated(cl, setup, schedule, case, args.steps, args.check_every, log=log)
+            res["validation"] = {"seed": args.val_seed, "history": vhist,
+                                 "chosen_worst": min(h["worst"] for h in vhist)}
+        else:
+            losses = []
+            for n, (lr, iters) in enumerate(schedule):
+                losses += list(cl.train(iters, lr=lr, seed=1 + n))
+        iters = len(losses)
         res["train_seconds"] = time.perf_counter() - t0
         l1 = float(np.mean([cl.loss(s) for s in range(cl.num_samples())]))
-        log(f"trained {args.iters} iterations in {res['train_seconds']:.1f} s: loss {l0:.4f} -> {l1:.4f}")
-        res.update(loss_initial=l0, loss_final=l1, loss_curve=[float(v) for v in losses[:: max(1, args.iters // 20)]])
+        log(f"trained {iters} iterations in {res['train_seconds']:.1f} s: loss {l0:.4f} -> {l1:.4f}")
+        res.update(schedule=args.schedule, loss_initial=l0, loss_final=l1,
+                   loss_curve=[float(v) for v in losses[:: max(1, iters // 20)]])
         if args.out:
             cl.save(args.out)
-    res.update(evaluate(cl, setup, args.eval_seed, args.steps, args.threshold_factor, gws=not args.no_gw, log=log))
+    per_seed = {}
+    for seed in args.eval_seeds:
+        per_seed[str(seed)] = evaluate(cl, setup, seed, args.steps, args.threshold_factor, gws=not args.no_gw, log=log)
+        r = per_seed[str(seed)]
+        log(f"seed {seed}: field worst bin {r['closure_field_err_max']:.4f} (plain {r['plain_field_err_max']:.4f}), "
+            f"mean {r['closure_field_err_mean']:.4f} (plain {r['plain_field_err_mean']:.4f}), "
+            f"within 5% {r['closure_bins_within_5pct']:.3f}")
+    res["eval"] = per_seed
+    res["summary"] = summarize(per_seed, gws=not args.no_gw)
     print(json.dumps(res))
 
 

@@ -365,15 +365,15 @@ __global__ __launch_bounds__(256) void k_lap_apply(const float *u, int N, float 
     v1[s] += dts1 * ia2 * y.y;
 }
 
-/* loss terms (lap u - t)^2 (block partials) and dL/d(lap u) = (lap u - t) / N^3 */
-__global__ __launch_bounds__(256) void k_lap_loss(const float *u, const float *t, int N, float *gy, double *part)
+/* loss terms (w - t)^2 (block partials) and dL/dw = (w - t) / N^3 for
+ * w = lap u [N3][2] and targets t [2][N3] */
+__global__ __launch_bounds__(256) void k_w_loss(const float *w, const float *t, long N3, float *gy, double *part)
 {
     __shared__ double red[4];
-    const long N2 = (long)N * N, N3 = N2 * N;
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc = 0.;
     if (s < N3) {
-        const float2 y = lap2(u, s, N, N2);
+        const float2 y = *(const float2 *)(w + 2 * s);
         const float d0 = y.x - t[s], d1 = y.y - t[N3 + s];
         acc = (double)d0 * d0 + (double)d1 * d1;
         const float inv = (float)(1. / (double)N3);
@@ -639,6 +639,16 @@ __global__ __launch_bounds__(256) void k_lap_field(const float *f, int N, float 
     out[N3 + s] = lap_soa(f + N3, s, N, N2) * inv_dx2;
 }
 
+/* w-target t_c <- (t_c - f_c) * scale_c: lap^-1 of the restricted fine
+ * Laplacian minus the restricted field, i.e. lap^-1 of the force target */
+__global__ __launch_bounds__(256) void k_sub_scale2(const float *f, long N3, float sc0, float sc1, float *t)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N3) return;
+    t[s] = (t[s] - f[s]) * sc0;
+    t[N3 + s] = (t[N3 + s] - f[N3 + s]) * sc1;
+}
+
 /* target t_c <- (t_c - lap f_c / dx^2) * scale_c: the restricted fine
  * Laplacian minus the coarse Laplacian of the restricted field */
 __global__ __launch_bounds__(256) void k_lap_target(const float *f, int N, float inv_dx2, float sc0, float sc1, float *t)
@@ -648,14 +658,6 @@ __global__ __launch_bounds__(256) void k_lap_target(const float *f, int N, float
     if (s >= N3) return;
     t[s] = (t[s] - lap_soa(f, s, N, N2) * inv_dx2) * sc0;
     t[N3 + s] = (t[N3 + s] - lap_soa(f + N3, s, N, N2) * inv_dx2) * sc1;
-}
-
-__global__ __launch_bounds__(256) void k_scale2(float *t, long N3, float sc0, float sc1)
-{
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N3) return;
-    t[s] *= sc0;
-    t[N3 + s] *= sc1;
 }
 
 __global__ void k_sumsq(const float *x, long n, double *part)
@@ -694,9 +696,12 @@ __global__ void k_gate(const DevScalars *sc, const double *hist, int hist_cap, d
 }
 
 /* coarse half-spectrum <- fine half-spectrum on |n_a| < Nc/2, times scale;
- * zero on the coarse Nyquist planes */
+ * zero on the coarse Nyquist planes.  With inv_lap set, each mode is also
+ * divided by the coarse lattice's Laplacian eigenvalue
+ * -(4/dx^2) sum_a sin^2(pi n_a / Nc) (inv_lap = dx^2; zero for n = 0): the
+ * restriction of lap^-1 of the array. */
 template <typename T>
-__global__ void k_restrict_modes(const T *fine, int Nf, T *coarse, int Nc, T scale)
+__global__ void k_restrict_modes(const T *fine, int Nf, T *coarse, int Nc, T scale, double inv_lap = 0.)
 {
     const int Hc = Nc / 2 + 1, Hf = Nf / 2 + 1;
     const long Mc = (long)Nc * Nc * Hc;
@@ -709,6 +714,13 @@ __global__ void k_restrict_modes(const T *fine, int Nf, T *coarse, int Nc, T sca
         const long mf = ((long)i_f * Nf + j_f) * Hf + kc;
         re = fine[2 * mf] * scale;
         im = fine[2 * mf + 1] * scale;
+        if (inv_lap != 0.) {
+            const double si = sin(M_PI * ic / Nc), sj = sin(M_PI * jc / Nc), sk = sin(M_PI * kc / Nc);
+            const double lam = -4. * (si * si + sj * sj + sk * sk);
+            const T w = (ic | jc | kc) == 0 ? (T)0 : (T)(inv_lap / lam);
+            re *= w;
+            im *= w;
+        }
     }
     coarse[2 * m] = re;
     coarse[2 * m + 1] = im;
@@ -860,15 +872,16 @@ int forward_loss(gabe_closure *cl, const bf16 *x, const float *t)
     a = conv_base(cl->N, cl->a2, cl->pk + PK_F3, cl->bias + 32);
     a.y_out = cl->ybuf;
     CHK((launch_conv<16, EPI_OUT>(a, cl->stream)));
-    /* y = lap lap u; dL/du = lap lap dL/dy (lap is self-adjoint) */
+    /* the loss compares w = lap u with targets for lap u (the force targets
+     * carried through lap^-1, so that every wavelength weighs by the relative
+     * error of the frequency correction it gets); dL/du = lap dL/dw (lap is
+     * self-adjoint) */
     const unsigned nb = nblocks(cl->N3, 256);
     hipLaunchKernelGGL(k_lap_out, dim3(nb), dim3(256), 0, cl->stream, cl->ybuf, cl->N, cl->wbuf);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_lap_loss, dim3(nb), dim3(256), 0, cl->stream, cl->wbuf, t, cl->N, cl->gybuf, cl->lpart);
+    hipLaunchKernelGGL(k_w_loss, dim3(nb), dim3(256), 0, cl->stream, cl->wbuf, t, cl->N3, cl->gybuf, cl->lpart);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_lap_out, dim3(nb), dim3(256), 0, cl->stream, cl->gybuf, cl->N, cl->wbuf);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_lap_grad, dim3(nb), dim3(256), 0, cl->stream, cl->wbuf, cl->N, cl->g3);
+    hipLaunchKernelGGL(k_lap_grad, dim3(nb), dim3(256), 0, cl->stream, cl->gybuf, cl->N, cl->g3);
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
@@ -966,18 +979,21 @@ int create_closure(gabe_closure *cl, int N, double dt, unsigned seed)
 bool fp32_pair(const gabe_ctx *c) { return c->fp32 && c->nf == 2; }
 
 /* restriction of one array of the fine context into one of the coarse */
-int restrict_array(gabe_ctx *f, gabe_ctx *c, void *src, void *dst)
+/* restriction of the fine array src onto the coarse array dst; with
+ * inverse_laplacian, of lap_coarse^-1 of it (zero mean) */
+int restrict_array(gabe_ctx *f, gabe_ctx *c, void *src, void *dst, bool inverse_laplacian = false)
 {
     CHK(run_plan(f, f->fft_rx, src, f->rx_buf));
     HIPCHK(hipStreamSynchronize(f->stream));
     const long Mc = (long)c->N * c->N * (c->N / 2 + 1);
     const double scale = 1.0 / (double)f->N3;
+    const double inv_lap = inverse_laplacian ? c->dx * c->dx : 0.;
     if (c->fp32)
         hipLaunchKernelGGL(k_restrict_modes<float>, dim3(nblocks(Mc, 256)), dim3(256), 0, c->stream,
-                           (const float *)f->rx_buf, f->N, (float *)c->rx_buf, c->N, (float)scale);
+                           (const float *)f->rx_buf, f->N, (float *)c->rx_buf, c->N, (float)scale, inv_lap);
     else
         hipLaunchKernelGGL(k_restrict_modes<double>, dim3(nblocks(Mc, 256)), dim3(256), 0, c->stream,
-                           (const double *)f->rx_buf, f->N, (double *)c->rx_buf, c->N, scale);
+                           (const double *)f->rx_buf, f->N, (double *)c->rx_buf, c->N, scale, inv_lap);
     HIPCHK(hipGetLastError());
     CHK(run_plan(c, c->fft_rx, c->rx_buf, dst));
     return GABE_OK;
@@ -1273,25 +1289,28 @@ int gabe_closure_sample(gabe_closure *cl, gabe_ctx *fine, gabe_ctx *coarse, int 
                        (const float *)fine->fbuf[fine->cur], fine->N, (float)(1. / (fine->dx * fine->dx)), cl->flap);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(fine->stream));
-    for (int c = 0; c < 2; c++) CHK(restrict_array(fine, coarse, cl->flap + c * N3f, t + c * N3));
-    const bool first = !cl->scaled;
-    const float s0 = first ? 1.f : (float)(1. / cl->s_out[0]), s1 = first ? 1.f : (float)(1. / cl->s_out[1]);
-    hipLaunchKernelGGL(k_lap_target, dim3(nblocks(N3, 256)), dim3(256), 0, coarse->stream, cf, cl->N,
-                       (float)(1. / (coarse->dx * coarse->dx)), s0, s1, t);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(coarse->stream));
-    HIPCHK(hipStreamSynchronize(cl->stream));
-    if (first) {   /* output scales: rms of the first raw target */
+    if (!cl->scaled) {   /* output scales: rms of the first raw force target */
+        for (int c = 0; c < 2; c++) CHK(restrict_array(fine, coarse, cl->flap + c * N3f, t + c * N3));
+        hipLaunchKernelGGL(k_lap_target, dim3(nblocks(N3, 256)), dim3(256), 0, coarse->stream, cf, cl->N,
+                           (float)(1. / (coarse->dx * coarse->dx)), 1.f, 1.f, t);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(coarse->stream));
         for (int c = 0; c < 2; c++) {
             CHK(rms_of(cl, t + c * N3, N3, &cl->s_out[c]));
             if (!(cl->s_out[c] > 0.)) return fail(GABE_ERR_STATE, "zero target in output channel %d", c);
         }
-        hipLaunchKernelGGL(k_scale2, dim3(nblocks(N3, 256)), dim3(256), 0, cl->stream, t, N3,
-                           (float)(1. / cl->s_out[0]), (float)(1. / cl->s_out[1]));
-        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(cl->stream));
         cl->scaled = true;
     }
+    /* target of w = lap u (unit 7-point stencil): lap_unit^-1 (force target / s_out)
+     * = (lap_c^-1 R(lap_fine f) - R f) / (dx_c^2 s_out), the restriction
+     * carrying lap_c^-1 in Fourier space */
+    for (int c = 0; c < 2; c++) CHK(restrict_array(fine, coarse, cl->flap + c * N3f, t + c * N3, true));
+    const double dx2 = coarse->dx * coarse->dx;
+    hipLaunchKernelGGL(k_sub_scale2, dim3(nblocks(N3, 256)), dim3(256), 0, coarse->stream, cf, N3,
+                       (float)(1. / (dx2 * cl->s_out[0])), (float)(1. / (dx2 * cl->s_out[1])), t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(coarse->stream));
     /* advance the fine lattice; the coarse one takes its restriction */
     if (fine_steps > 0) {
         CHK(gabe_step(fine, fine_steps));

@@ -32,6 +32,7 @@ EXPORTS = (
     "gabe_spectra", "gabe_add_fluctuation", "gabe_effmass", "gabe_tt_source",
     "gabe_history", "gabe_kernel_times", "gabe_query", "gabe_autotune_sweep", "gabe_mode_tiles",
     "gabe_ensemble_create", "gabe_ensemble_member", "gabe_ensemble_step", "gabe_ensemble_sync",
+    "gabe_ensemble_kernel_times",
     "gabe_ensemble_destroy",
     "gabe_restrict", "gabe_closure_create", "gabe_closure_destroy", "gabe_closure_params",
     "gabe_closure_set_params", "gabe_closure_scales", "gabe_closure_set_scales", "gabe_closure_sample",
@@ -109,6 +110,7 @@ def load():
         "gabe_ensemble_member": [vp, ip, C.POINTER(vp)],
         "gabe_ensemble_step": [vp, ip],
         "gabe_ensemble_sync": [vp],
+        "gabe_ensemble_kernel_times": [vp, ip, dp],
         "gabe_ensemble_destroy": [vp],
         "gabe_restrict": [vp, vp],
         "gabe_closure_create": [ip, C.c_double, C.c_uint, C.POINTER(vp)],
@@ -363,6 +365,12 @@ class Ensemble:
 
     def sync(self):
         _check(load().gabe_ensemble_sync(self._h))
+
+    def kernel_times(self, nsteps):
+        """mean ms of the batched sweep A, finalize and sweep B over nsteps steps"""
+        out = np.zeros(3)
+        _check(load().gabe_ensemble_kernel_times(self._h, nsteps, _dp(out)))
+        return dict(zip(("sweep_a", "finalize", "sweep_b"), out.tolist()))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

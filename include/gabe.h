@@ -227,12 +227,18 @@ int gabe_mode_tiles(int N, int precision);
  * other call (upload, initialise, scalars, spectra, download) and give
  * bit-identical results to the same lattice stepped alone.  All members share
  * precision and nflds; tensor modes are not batched (calc_gws must be 0).
+ * Members created with planes_per_block = 0 march min(N, 32) planes per
+ * workgroup (the batched launch fills the GPU without smaller tiles).
  * No reference counterpart: the reference runs one lattice per process. */
 typedef struct gabe_ensemble gabe_ensemble;
 int gabe_ensemble_create(const gabe_params *params, int count, gabe_ensemble **out);
 int gabe_ensemble_member(gabe_ensemble *ens, int idx, gabe_ctx **out);
 int gabe_ensemble_step(gabe_ensemble *ens, int nsteps);
 int gabe_ensemble_sync(gabe_ensemble *ens);
+/* Timing hook: nsteps ensemble steps launched one by one (not from the graph)
+ * with HIP events around each batched phase on the ensemble's stream; writes
+ * the mean ms of the batched sweep A, finalize and sweep B to ms_out[0..3). */
+int gabe_ensemble_kernel_times(gabe_ensemble *ens, int nsteps, double *ms_out);
 int gabe_ensemble_destroy(gabe_ensemble *ens);
 
 /* ------------------------------------------------------------------------
@@ -250,14 +256,18 @@ int gabe_ensemble_destroy(gabe_ensemble *ens);
  * by the plain method: the state of a fine lattice spectrally restricted onto
  * the coarse one, and as target what the coarse lattice's Laplacian misses of
  * the fine one, R(lap_fine f) - lap_coarse(R f) with lap the 7-point
- * Laplacian over dx^2 and R the restriction.  Lattices with a closure: FP32,
+ * Laplacian over dx^2 and R the restriction (stored, for the loss, as the
+ * target of lap u: lap_unit^-1 of it over s_out).  Lattices with a closure: FP32,
  * two fields, N in {32, 64, 96, 128}.
  *
  * Parameters, FP32, GABE_CLOSURE_NPARAMS of them in this order:
  *   W1[16][4][27], b1[16], W2[16][16][27], b2[16], W3[2][16][27], b3[2]
  * with tap t = 9 (dz+1) + 3 (dy+1) + (dx+1), dz along the slowest index.
- * Loss: sum over sites s and channels c of (y - t)^2 / (2 N^3).  (b3 drops
- * out of y: its gradient is zero.)
+ * Loss: sum over sites s and channels c of (lap u - t)^2 / (2 N^3), t the
+ * target of lap u: the force target carried through lap^-1, so each
+ * wavelength weighs by the relative error of the frequency correction it
+ * gets rather than by the size of the force.  (b3 drops out of lap u: its
+ * gradient is zero.)
  * ---------------------------------------------------------------------- */
 #define GABE_CLOSURE_NPARAMS 9538
 typedef struct gabe_closure gabe_closure;
@@ -280,7 +290,8 @@ int gabe_closure_set_scales(gabe_closure *cl, const double *s_in, const double *
 
 /* One training pair from the plain method.  With restrict_first the coarse
  * context first takes the restriction of the fine one; the input is the
- * coarse state, the target R(lap_fine f) - lap_coarse(R f) per field.  Then
+ * coarse state, the target R(lap_fine f) - lap_coarse(R f) per field (over
+ * s_out, carried through the unit stencil's inverse: the target of lap u).  Then
  * the fine context takes fine_steps plain steps (0: none) and the coarse
  * context the restriction of the advanced state, ready for the next call with
  * restrict_first = 0.  Returns the sample index. */
@@ -296,7 +307,7 @@ int gabe_closure_loss(gabe_closure *cl, int sample, double *loss);
 /* Network on host data in scaled units: x[N^3][4] -> y = lap lap u, [N^3][2]. */
 int gabe_closure_forward(gabe_closure *cl, const float *x, float *y);
 /* Loss and its gradient with respect to the parameters for host data
- * x[N^3][4] and targets t[2][N^3] (the kernels training uses). */
+ * x[N^3][4] and targets t[2][N^3] of lap u (the kernels training uses). */
 int gabe_closure_gradient(gabe_closure *cl, const float *x, const float *t, float *grad, double *loss);
 
 /* Apply the closure after every step of ctx (weights and scales are copied:

@@ -8,7 +8,8 @@ reference; its definition is the one the C ABI documents:
     u  = conv3(a2; W3) + b3,        y = lap lap u (lap u = 6 neighbours - 6 u),
     conv3(u; W)[s, co] = sum_t sum_ci W[co, ci, t] u[s + off(t), ci],
     off(t) = (t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1) on a periodic N^3 grid,
-    loss   = sum_{s,c} (y - target)^2 / (2 N^3).
+    loss   = sum_{s,c} (lap u - target)^2 / (2 N^3),
+the target being that of lap u (the force target carried through lap^-1).
 
 With bf16=True the rounding points of the device path are reproduced (inputs,
 weights, stored pre-activations, activations and back-propagated gradients
@@ -95,8 +96,9 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def forward(p, x, bf16=False):
-    """x [N,N,N,4] -> y [N,N,N,2] and the cache of the backward pass."""
+def forward_u(p, x, bf16=False):
+    """x [N,N,N,4] -> the network output u [N,N,N,2] and the cache of the
+    backward pass."""
     P = unpack(p)
     rd = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     W1, W2, W3 = rd(P["W1"]), rd(P["W2"]), rd(P["W3"])
@@ -107,19 +109,28 @@ def forward(p, x, bf16=False):
     z2 = conv3(a1, W2) + P["b2"]
     a2 = rd(silu(z2))
     z2 = rd(z2)
-    y = lap(lap(conv3(a2, W3) + P["b3"]))
-    return y, dict(x=x, z1=z1, a1=a1, z2=z2, a2=a2, W1=W1, W2=W2, W3=W3)
+    u = conv3(a2, W3) + P["b3"]
+    return u, dict(x=x, z1=z1, a1=a1, z2=z2, a2=a2, W1=W1, W2=W2, W3=W3)
+
+
+def forward(p, x, bf16=False):
+    """x [N,N,N,4] -> y = lap lap u [N,N,N,2] (the correction applied to the
+    velocities) and the cache of the backward pass."""
+    u, c = forward_u(p, x, bf16)
+    return lap(lap(u)), c
 
 
 def loss_and_grad(p, x, target, bf16=False):
-    """target [2][N,N,N] (channel-major, as the device stores it)."""
-    y, c = forward(p, x, bf16)
+    """Loss sum (lap u - t)^2 / (2 N^3) and its gradient; target [2][N,N,N]
+    (channel-major, as the device stores it) is the target of lap u."""
+    u, c = forward_u(p, x, bf16)
+    w = lap(u)
     rd = bf16_round if bf16 else (lambda a: a)
-    n3 = y.shape[0] * y.shape[1] * y.shape[2]
+    n3 = w.shape[0] * w.shape[1] * w.shape[2]
     t = np.moveaxis(np.asarray(target, dtype=np.float64), 0, -1)
-    d = y - t
+    d = w - t
     loss = float((d * d).sum() / (2.0 * n3))
-    gy = rd(lap(lap(d / n3)))      # dL/du: lap is self-adjoint
+    gy = rd(lap(d / n3))      # dL/du: lap is self-adjoint
     g = {}
     g["W3"] = weight_grad(gy, c["a2"])
     g["b3"] = gy.reshape(-1, NOUT).sum(0)
@@ -133,7 +144,8 @@ def loss_and_grad(p, x, target, bf16=False):
 
 
 def loss_only(p, x, target):
-    y, _ = forward(p, x)
+    u, _ = forward_u(p, x)
+    w = lap(u)
     t = np.moveaxis(np.asarray(target, dtype=np.float64), 0, -1)
-    n3 = y.shape[0] * y.shape[1] * y.shape[2]
-    return float(((y - t) ** 2).sum() / (2.0 * n3))
+    n3 = w.shape[0] * w.shape[1] * w.shape[2]
+    return float(((w - t) ** 2).sum() / (2.0 * n3))

@@ -65,7 +65,7 @@ def test_gradient_matches_cpu_network(net):
     lref, gref = cr.loss_and_grad(p, x, t, bf16=True)
     assert abs(loss - lref) <= 1e-4 * lref
     G, R = tensors(g.astype(np.float64)), tensors(gref)
-    # b3 drops out of y = lap lap u: its gradient is the sum of N^3 BF16-rounded
+    # b3 drops out of lap u: its gradient is the sum of N^3 BF16-rounded
     # terms of zero sum, i.e. rounding noise
     assert np.abs(G["b3"]).max() <= 2e-2 * np.abs(G["b2"]).max()
     for name in G:

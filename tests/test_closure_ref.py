@@ -5,7 +5,7 @@
 * conv3_adjoint is the adjoint of conv3: <conv3(u), g> = <u, conv3_adjoint(g)>;
 * the analytic gradient equals central finite differences of the FP64 loss
   (every bias and a random subset of every weight tensor); b3 drops out of
-  y = lap lap u, so its gradient is zero;
+  lap u, so its gradient is zero;
 * bf16_round rounds to nearest with ties to even.
 """
 import numpy as np
