@@ -60,16 +60,22 @@ static_assert(P_TOTAL == GABE_CLOSURE_NPARAMS, "parameter count of the C ABI");
 constexpr int TAP_COLS = 9;
 constexpr int WIDE_STEPS = 5;    /* 9 columns x 16 channels, 2 columns per MFMA (the last half empty) */
 constexpr int NARROW_STEPS = 2;  /* 9 columns x 4 channels, 8 columns per MFMA (the second step one column) */
+/* The output layer has 2 channels, too few to fill 16 MFMA rows; there the
+ * rows are (plane of a wave's pair, output channel, dy) and K runs over 12
+ * (plane, dx) columns of 16 channels: the four input planes the wave's two
+ * output planes read, so one MFMA step per halo row covers every tap. */
+constexpr int OUT_STEPS = 6;
 constexpr int WG_ROWS = 2;       /* lattice rows per weight-gradient slab */
 constexpr int WG_SLABS = 8;      /* slabs per weight-gradient workgroup */
 
 enum ConvEpi { EPI_HIDDEN_TRAIN, EPI_HIDDEN, EPI_BWD, EPI_OUT };
 
-/* packed MFMA weight operands, [steps][3 dy][64 lanes] x 8 BF16: the three
- * forward layers, then the backward-data convolutions of layers 3 and 2;
- * biases are kept as [3][16] FP32, zero padded */
+/* packed MFMA weight operands, [steps][3 dy][64 lanes] x 8 BF16 ([steps][64
+ * lanes] for the output layer): the three forward layers, then the
+ * backward-data convolutions of layers 3 and 2; biases are kept as [3][16]
+ * FP32, zero padded */
 constexpr size_t PK_F1 = 0, PK_F2 = PK_F1 + NARROW_STEPS * 3 * 64, PK_F3 = PK_F2 + WIDE_STEPS * 3 * 64,
-                 PK_B3 = PK_F3 + WIDE_STEPS * 3 * 64, PK_B2 = PK_B3 + NARROW_STEPS * 3 * 64,
+                 PK_B3 = PK_F3 + OUT_STEPS * 64, PK_B2 = PK_B3 + NARROW_STEPS * 3 * 64,
                  PK_TOTAL = PK_B2 + WIDE_STEPS * 3 * 64;
 
 struct ConvArgs {
@@ -126,7 +132,10 @@ __device__ __forceinline__ int wrap(int x, int N) { return x < 0 ? x + N : (x >=
 template <int CI, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv(ConvArgs A)
 {
-    constexpr int NSTEP = CI == 16 ? WIDE_STEPS : NARROW_STEPS;
+    constexpr bool OUTL = EPI == EPI_OUT;
+    static_assert(!OUTL || (CI == 16 && ZW == 2), "the output layer reads 16 channels, two planes per wave");
+    constexpr int NSTEP = OUTL ? OUT_STEPS : (CI == 16 ? WIDE_STEPS : NARROW_STEPS);
+    constexpr int NDY = OUTL ? 1 : 3;             /* weight operands per step */
     constexpr int CH = CI == 16 ? 2 : 1;          /* copy chunks per site: 16-byte halves, or one 8-byte site */
     constexpr int CE = CI / CH;                   /* bf16 per chunk */
     using Chunk = typename std::conditional<CI == 16, bf16x8, bf16x4>::type;
@@ -187,17 +196,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     const int N2CI = N * N * CI;
 
-    bf16x8 wr[NSTEP][3];
+    bf16x8 wr[NSTEP][NDY];
 #pragma unroll
     for (int s = 0; s < NSTEP; s++)
 #pragma unroll
-        for (int dy = 0; dy < 3; dy++) wr[s][dy] = A.w[(s * 3 + dy) * 64 + lane];
+        for (int dy = 0; dy < NDY; dy++) wr[s][dy] = A.w[(s * NDY + dy) * 64 + lane];
+    /* bias of the lane's result channels: 4g .. 4g + 3, or in the output
+     * layer output channel g & 1 */
     float bias[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) bias[q] = A.bias ? A.bias[4 * g + q] : 0.f;
+    for (int q = 0; q < 4; q++) bias[q] = A.bias ? A.bias[OUTL ? (g & 1) : 4 * g + q] : 0.f;
 
     /* this lane's K slots: per step one tap column (8 of its 16 channels) or
-     * two (4 channels each): the column's dz, and the offset of its dx and
+     * two (4 channels each): the column's dz (in the output layer the plane
+     * 0..3 of the wave's four input planes), and the offset of its dx and
      * channels within a plane slot.  Slots past the 9 columns carry zero
      * weights; they read the last column, so no operand needs a select. */
     constexpr int NC = CI == 16 ? 1 : 2;
@@ -207,7 +219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             int col = CI == 16 ? 2 * s + (g >> 1) : 8 * s + 2 * g + c;
-            col = col < TAP_COLS ? col : TAP_COLS - 1;
+            if (!OUTL) col = col < TAP_COLS ? col : TAP_COLS - 1;
             cdz[s][c] = col / 3;
             coff[s][c] = (r + col % 3) * CI + (CI == 16 ? 8 * (g & 1) : 0);
         }
@@ -229,81 +241,114 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     pre[m] = fetch(__umul24(ii, N2CI) + cjk[m]);
                 }
         }
+        if constexpr (OUTL) {
+            /* the wave's output planes z0, z0 + 1 read input planes z0 - 1 ..
+             * z0 + 2 (slots of plane offsets z0 .. z0 + 3); per halo row yy one
+             * chain of NSTEP MFMAs gives P[yy]: lane (r, g) holds, for output
+             * channel g & 1 of plane z0 + (g >> 1), the partial sums of dy = 0,
+             * 1, 2 over that row, and output row y sums halo rows y .. y + 2 */
+            const int z0 = ZW * wave;
+            int ad[NSTEP];
 #pragma unroll
-        for (int zp = 0; zp < ZW; zp++) {
-            const int z = ZW * wave + zp;
-            /* plane slots of this wave's planes i = ib + TZ n + z + dz - 1 */
-            int slot[3];
-#pragma unroll
-            for (int dz = 0; dz < 3; dz++) slot[dz] = ((TZ * n + z + dz) % RING) * PLANE * CI;
-            int ad[NSTEP][NC];
-#pragma unroll
-            for (int s = 0; s < NSTEP; s++)
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    const int d = cdz[s][c];
-                    ad[s][c] = (d == 0 ? slot[0] : (d == 1 ? slot[1] : slot[2])) + coff[s][c];
-                }
-            /* the activation operands of halo row yy + 1 are read from LDS while
-             * the MFMAs of row yy run */
-            auto read_row = [&](bf16x8 *av, int yy) {
-#pragma unroll
-                for (int s = 0; s < NSTEP; s++) {
-                    if constexpr (CI == 16) {
-                        av[s] = *(const bf16x8 *)(tin + ad[s][0] + yy * HX * 16);
-                    } else {
-                        const bf16x4 lo = *(const bf16x4 *)(tin + ad[s][0] + yy * HX * 4);
-                        const bf16x4 hi = *(const bf16x4 *)(tin + ad[s][1] + yy * HX * 4);
-                        av[s] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
-                    }
-                }
-            };
+            for (int s = 0; s < NSTEP; s++) ad[s] = ((TZ * n + z0 + cdz[s][0]) % RING) * PLANE * CI + coff[s][0];
             bf16x8 avc[NSTEP], avn[NSTEP];
-            read_row(avc, 0);
-            f32x4 acc[TY];
 #pragma unroll
-            for (int y = 0; y < TY; y++) acc[y] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int s = 0; s < NSTEP; s++) avc[s] = *(const bf16x8 *)(tin + ad[s]);
+            f32x4 P[TY + 2];
 #pragma unroll
             for (int yy = 0; yy < TY + 2; yy++) {
-                if (yy + 1 < TY + 2) read_row(avn, yy + 1);
+                if (yy + 1 < TY + 2) {
 #pragma unroll
-                for (int s = 0; s < NSTEP; s++)
+                    for (int s = 0; s < NSTEP; s++) avn[s] = *(const bf16x8 *)(tin + ad[s] + (yy + 1) * HX * 16);
+                }
+                f32x4 p = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int dy = 0; dy < 3; dy++) {
-                        const int y = yy - dy;
-                        if (y >= 0 && y < TY)
-                            acc[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[s][dy], avc[s], acc[y], 0, 0, 0);
-                    }
+                for (int s = 0; s < NSTEP; s++) p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[s][0], avc[s], p, 0, 0, 0);
+                P[yy] = p;
 #pragma unroll
                 for (int s = 0; s < NSTEP; s++) avc[s] = avn[s];
             }
-            /* lane holds D[channels 4g .. 4g + 3][site k0 + r of row (i, j0 + y)] */
-            const int i = ib + TZ * n + z;
+            const int i = ib + TZ * n + z0 + (g >> 1);
 #pragma unroll
             for (int y = 0; y < TY; y++) {
                 const int site = __umul24(__umul24(i, N) + j0 + y, N) + k0 + r;
-                float v[4];
+                A.y_out[site * 2 + (g & 1)] = (P[y][0] + P[y + 1][1]) + P[y + 2][2] + bias[0];
+            }
+        } else {
 #pragma unroll
-                for (int q = 0; q < 4; q++) v[q] = acc[y][q] + bias[q];
-                if constexpr (EPI == EPI_HIDDEN_TRAIN || EPI == EPI_HIDDEN) {
-                    bf16x4 h;
+            for (int zp = 0; zp < ZW; zp++) {
+                const int z = ZW * wave + zp;
+                /* plane slots of this wave's planes i = ib + TZ n + z + dz - 1 */
+                int slot[3];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) h[q] = (bf16)silu(v[q]);
-                    *(bf16x4 *)(A.a_out + site * 16 + 4 * g) = h;
-                    if constexpr (EPI == EPI_HIDDEN_TRAIN) {
-                        bf16x4 zq;
+                for (int dz = 0; dz < 3; dz++) slot[dz] = ((TZ * n + z + dz) % RING) * PLANE * CI;
+                int ad[NSTEP][NC];
 #pragma unroll
-                        for (int q = 0; q < 4; q++) zq[q] = (bf16)v[q];
-                        *(bf16x4 *)(A.z_out + site * 16 + 4 * g) = zq;
+                for (int s = 0; s < NSTEP; s++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++) {
+                        const int d = cdz[s][c];
+                        ad[s][c] = (d == 0 ? slot[0] : (d == 1 ? slot[1] : slot[2])) + coff[s][c];
                     }
-                } else if constexpr (EPI == EPI_BWD) {
-                    const bf16x4 zq = *(const bf16x4 *)(A.zin + site * 16 + 4 * g);
-                    bf16x4 o;
+                /* the activation operands of halo row yy + 1 are read from LDS while
+                 * the MFMAs of row yy run */
+                auto read_row = [&](bf16x8 *av, int yy) {
 #pragma unroll
-                    for (int q = 0; q < 4; q++) o[q] = (bf16)(acc[y][q] * dsilu((float)zq[q]));
-                    *(bf16x4 *)(A.g_out + site * 16 + 4 * g) = o;
-                } else if constexpr (EPI == EPI_OUT) {
-                    if (g == 0) *(float2 *)(A.y_out + site * 2) = make_float2(v[0], v[1]);
+                    for (int s = 0; s < NSTEP; s++) {
+                        if constexpr (CI == 16) {
+                            av[s] = *(const bf16x8 *)(tin + ad[s][0] + yy * HX * 16);
+                        } else {
+                            const bf16x4 lo = *(const bf16x4 *)(tin + ad[s][0] + yy * HX * 4);
+                            const bf16x4 hi = *(const bf16x4 *)(tin + ad[s][1] + yy * HX * 4);
+                            av[s] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                        }
+                    }
+                };
+                bf16x8 avc[NSTEP], avn[NSTEP];
+                read_row(avc, 0);
+                f32x4 acc[TY];
+#pragma unroll
+                for (int y = 0; y < TY; y++) acc[y] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int yy = 0; yy < TY + 2; yy++) {
+                    if (yy + 1 < TY + 2) read_row(avn, yy + 1);
+#pragma unroll
+                    for (int s = 0; s < NSTEP; s++)
+#pragma unroll
+                        for (int dy = 0; dy < 3; dy++) {
+                            const int y = yy - dy;
+                            if (y >= 0 && y < TY)
+                                acc[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[s][dy], avc[s], acc[y], 0, 0, 0);
+                        }
+#pragma unroll
+                    for (int s = 0; s < NSTEP; s++) avc[s] = avn[s];
+                }
+                /* lane holds D[channels 4g .. 4g + 3][site k0 + r of row (i, j0 + y)] */
+                const int i = ib + TZ * n + z;
+#pragma unroll
+                for (int y = 0; y < TY; y++) {
+                    const int site = __umul24(__umul24(i, N) + j0 + y, N) + k0 + r;
+                    float v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = acc[y][q] + bias[q];
+                    if constexpr (EPI == EPI_HIDDEN_TRAIN || EPI == EPI_HIDDEN) {
+                        bf16x4 h;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) h[q] = (bf16)silu(v[q]);
+                        *(bf16x4 *)(A.a_out + site * 16 + 4 * g) = h;
+                        if constexpr (EPI == EPI_HIDDEN_TRAIN) {
+                            bf16x4 zq;
+#pragma unroll
+                            for (int q = 0; q < 4; q++) zq[q] = (bf16)v[q];
+                            *(bf16x4 *)(A.z_out + site * 16 + 4 * g) = zq;
+                        }
+                    } else if constexpr (EPI == EPI_BWD) {
+                        const bf16x4 zq = *(const bf16x4 *)(A.zin + site * 16 + 4 * g);
+                        bf16x4 o;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) o[q] = (bf16)(acc[y][q] * dsilu((float)zq[q]));
+                        *(bf16x4 *)(A.g_out + site * 16 + 4 * g) = o;
+                    }
                 }
             }
         }
@@ -563,23 +608,38 @@ __device__ float packed_weight(const float *W, int wco, int wci, bool flip, int 
     return W[(ci * wci + n) * CL_TAPS + (CL_TAPS - 1 - t)];
 }
 
+/* Output-layer operand of step s: row n = lane & 15 is (plane zp = n >> 3 of
+ * the wave's pair, output channel co = (n >> 2) & 1, dy = n & 3, dy = 3
+ * empty); K chunk g = lane >> 4 is the column 2 s + (g >> 1) = 3 pl + dx of
+ * input plane pl (0..3) with channels 8 (g & 1) .. + 7, the tap dz = pl - zp. */
+__device__ float out_weight(const float *W3, int s, int lane, int e)
+{
+    const int n = lane & 15, g = lane >> 4;
+    const int zp = n >> 3, co = (n >> 2) & 1, dy = n & 3;
+    const int col = 2 * s + (g >> 1), pl = col / 3, dx = col % 3, ci = 8 * (g & 1) + e;
+    const int dz = pl - zp;
+    if (dy > 2 || dz < 0 || dz > 2) return 0.f;
+    return W3[(co * CL_H + ci) * CL_TAPS + 9 * dz + 3 * dy + dx];
+}
+
 __global__ void k_pack(const float *P, bf16x8 *pk, float *bias)
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < (long)PK_TOTAL) {
         const float *W;
-        int wco, wci, wide, s;
+        int wco, wci, wide, s;   /* wide: 0 four-channel steps, 1 sixteen-channel steps, 2 the output layer */
         bool flip;
         long off;
         if (q < (long)PK_F2) { W = P + P_W1; wco = 16; wci = 4; flip = false; wide = 0; off = PK_F1; }
         else if (q < (long)PK_F3) { W = P + P_W2; wco = 16; wci = 16; flip = false; wide = 1; off = PK_F2; }
-        else if (q < (long)PK_B3) { W = P + P_W3; wco = 2; wci = 16; flip = false; wide = 1; off = PK_F3; }
+        else if (q < (long)PK_B3) { W = P + P_W3; wco = 2; wci = 16; flip = false; wide = 2; off = PK_F3; }
         else if (q < (long)PK_B2) { W = P + P_W3; wco = 2; wci = 16; flip = true; wide = 0; off = PK_B3; }
         else { W = P + P_W2; wco = 16; wci = 16; flip = true; wide = 1; off = PK_B2; }
         s = (int)((q - off) / 64);
         const int lane = (int)((q - off) % 64);
         bf16x8 v;
-        for (int e = 0; e < 8; e++) v[e] = (bf16)packed_weight(W, wco, wci, flip, wide, s, lane, e);
+        for (int e = 0; e < 8; e++)
+            v[e] = (bf16)(wide == 2 ? out_weight(W, s, lane, e) : packed_weight(W, wco, wci, flip, wide, s, lane, e));
         pk[q] = v;
     }
     if (q < 48) {
