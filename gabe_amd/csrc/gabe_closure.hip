@@ -78,6 +78,39 @@ constexpr size_t PK_F1 = 0, PK_F2 = PK_F1 + NARROW_STEPS * 3 * 64, PK_F3 = PK_F2
                  PK_B3 = PK_F3 + OUT_STEPS * 64, PK_B2 = PK_B3 + NARROW_STEPS * 3 * 64,
                  PK_TOTAL = PK_B2 + WIDE_STEPS * 3 * 64;
 
+/* Friedmann-constraint gate (on the device, no host round trip): from the
+ * last three history rows, R = |rho_dot + 3H(rho + p)| / (3 H rho) at the
+ * middle row with p = K - G/3 - V; the closure stops for good once R exceeds
+ * the threshold or a value is not finite.  Records the largest R, and the
+ * step of the fallback, for the host. */
+struct GateArgs {
+    const DevScalars *sc;
+    const double *hist;
+    int hist_cap;
+    double thr;
+    int *gate;               /* [0] active */
+    long long *fallback;
+    double *rmax;
+};
+
+__device__ void gate_eval(const GateArgs &G)
+{
+    if (G.gate[0] == 0) return;
+    const long long n = G.sc->nstep - 1;
+    if (n - 2 < G.sc->hist0) return;   /* three rows of this trajectory are needed */
+    const double *rm = G.hist + ((n - 2) % G.hist_cap) * GABE_HIST_COLS;
+    const double *r0 = G.hist + ((n - 1) % G.hist_cap) * GABE_HIST_COLS;
+    const double *rp = G.hist + (n % G.hist_cap) * GABE_HIST_COLS;
+    const double H = r0[2] / r0[1], rho = r0[6], p = r0[3] - r0[5] / 3. - r0[4];
+    const double rhodot = (rp[6] - rm[6]) / (rp[0] - rm[0]);
+    const double R = fabs(rhodot + 3. * H * (rho + p)) / (3. * H * rho);
+    if (R > G.rmax[0]) G.rmax[0] = R;
+    if (!(R <= G.thr)) {
+        G.gate[0] = 0;
+        G.fallback[0] = G.sc->nstep;
+    }
+}
+
 struct ConvArgs {
     const bf16 *in;
     const bf16x8 *w;
@@ -89,6 +122,7 @@ struct ConvArgs {
     bf16 *g_out;             /* BWD: [N3][16] */
     float *y_out;            /* OUT: [N3][2] */
     const int *gate;         /* skip everything when gate[0] == 0 */
+    GateArgs gatev;          /* first layer after a step: evaluate the gate (gatev.gate set) */
     const float *fin[4];     /* 4-channel input as FP32 arrays (used instead of in when set) */
     float fscale[4];
 };
@@ -227,6 +261,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
      * only vector loads in flight inside the march are the next planes */
     __builtin_amdgcn_s_waitcnt(0x0F70);   /* vmcnt(0) */
     __syncthreads();
+    /* the first layer after a step evaluates the gate in one thread, after
+     * every thread of its workgroup has read the flag; the layers after it
+     * (and this layer's workgroups that read the flag later) skip once it
+     * closes, and what this layer wrote is then never read */
+    if constexpr (CI == 4 && EPI == EPI_HIDDEN)
+        if (A.gatev.gate && blockIdx.x == 0 && tid == 0) (void)gate_eval(A.gatev);
 
     const int nit = D / TZ;
     for (int n = 0; n < nit; n++) {
@@ -395,19 +435,49 @@ __device__ __forceinline__ float2 lap2(const float *u, long s, int N, long N2)
     return make_float2(sum.x - 6.f * c.x, sum.y - 6.f * c.y);
 }
 
-/* the correction after a step: v_c += dt s_out[c] (lap u)_c / a^2 */
-__global__ __launch_bounds__(256) void k_lap_apply(const float *u, int N, float *v0, float *v1, const DevScalars *sc,
+/* The correction after a step, v_c += dt s_out[c] (lap w)_c / a^2 with
+ * w = lap u, in one pass: a workgroup owns a KI x KJ x KK block of sites,
+ * computes w on the block and its 1-site halo into LDS (all of the u reads
+ * of the block in flight at once), then corrects the block from LDS.  Both
+ * Laplacians are lap2's arithmetic, site by site. */
+constexpr int KI = 4, KJ = 8, KK = 32, HJ = KJ + 2, HK = KK + 2, HP = HJ * HK, KH = (KI + 2) * HP;
+
+__device__ __forceinline__ float2 lap2_tile(const float2 *w, int h)
+{
+    const float2 c = w[h];
+    float2 sum = w[h + HP];
+    for (float2 v : {w[h - HP], w[h + HK], w[h - HK], w[h + 1], w[h - 1]}) {
+        sum.x += v.x;
+        sum.y += v.y;
+    }
+    return make_float2(sum.x - 6.f * c.x, sum.y - 6.f * c.y);
+}
+
+__global__ __launch_bounds__(256) void k_lap2_kick(const float *u, int N, float *v0, float *v1, const DevScalars *sc,
                                                    float dts0, float dts1, const int *gate)
 {
     if (gate && gate[0] == 0) return;
-    const long N2 = (long)N * N, N3 = N2 * N;
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N3) return;
+    __shared__ float2 w[KH];
+    const long N2 = (long)N * N;
+    const int tid = threadIdx.x, ntk = N / KK, ntj = N / KJ, wg = blockIdx.x;
+    const int k0 = (wg % ntk) * KK, j0 = ((wg / ntk) % ntj) * KJ, i0 = (wg / (ntk * ntj)) * KI;
+#pragma unroll 4
+    for (int h = tid; h < KH; h += 256) {
+        const int hi = h / HP, hr = h - hi * HP;
+        const int i = wrap(i0 + hi - 1, N), j = wrap(j0 + hr / HK - 1, N), k = wrap(k0 + hr % HK - 1, N);
+        w[h] = lap2(u, ((long)i * N + j) * N + k, N, N2);
+    }
+    __syncthreads();
     const double a = sc->a[0];
     const float ia2 = (float)(1. / (a * a));
-    const float2 y = lap2(u, s, N, N2);
-    v0[s] += dts0 * ia2 * y.x;
-    v1[s] += dts1 * ia2 * y.y;
+    const int tj = tid / KK, tk = tid % KK;
+#pragma unroll
+    for (int ti = 0; ti < KI; ti++) {
+        const float2 y = lap2_tile(w, (ti + 1) * HP + (tj + 1) * HK + tk + 1);
+        const long s = ((long)(i0 + ti) * N + j0 + tj) * N + k0 + tk;
+        v0[s] += dts0 * ia2 * y.x;
+        v1[s] += dts1 * ia2 * y.y;
+    }
 }
 
 /* loss terms (w - t)^2 (block partials) and dL/dw = (w - t) / N^3 for
@@ -732,29 +802,6 @@ __global__ void k_sumsq(const float *x, long n, double *part)
     if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-/* Friedmann-constraint gate (on the device, no host round trip): from the
- * last three history rows, R = |rho_dot + 3H(rho + p)| / (3 H rho) at the
- * middle row with p = K - G/3 - V; the closure stops for good once R exceeds
- * the threshold or a value is not finite. */
-__global__ void k_gate(const DevScalars *sc, const double *hist, int hist_cap, double thr, int *gate,
-                       long long *fallback, double *rmax)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0 || gate[0] == 0) return;
-    const long long n = sc->nstep - 1;
-    if (n - 2 < sc->hist0) return;   /* three rows of this trajectory are needed */
-    const double *rm = hist + ((n - 2) % hist_cap) * GABE_HIST_COLS;
-    const double *r0 = hist + ((n - 1) % hist_cap) * GABE_HIST_COLS;
-    const double *rp = hist + (n % hist_cap) * GABE_HIST_COLS;
-    const double H = r0[2] / r0[1], rho = r0[6], p = r0[3] - r0[5] / 3. - r0[4];
-    const double rhodot = (rp[6] - rm[6]) / (rp[0] - rm[0]);
-    const double R = fabs(rhodot + 3. * H * (rho + p)) / (3. * H * rho);
-    if (R > rmax[0]) rmax[0] = R;
-    if (!(R <= thr)) {
-        gate[0] = 0;
-        fallback[0] = sc->nstep;
-    }
-}
-
 /* coarse half-spectrum <- fine half-spectrum on |n_a| < Nc/2, times scale;
  * zero on the coarse Nyquist planes.  With inv_lap set, each mode is also
  * divided by the coarse lattice's Laplacian eigenvalue
@@ -841,7 +888,7 @@ struct ClosureRun {
     bf16x8 *pk = nullptr;
     float *bias = nullptr;
     bf16 *a1 = nullptr, *a2 = nullptr;
-    float *u = nullptr, *w = nullptr;   /* network output u and lap u, [N3][2] */
+    float *u = nullptr;            /* network output u, [N3][2] */
     int *gate = nullptr;           /* [0] active */
     long long *fallback = nullptr; /* step count when the gate closed */
     double *rmax = nullptr;
@@ -1079,7 +1126,6 @@ int run_create(gabe_ctx *c, const gabe_detail::ClosureCkpt &k, const void *pk, c
     if (hipMalloc(&R->pk, sizeof(bf16x8) * PK_TOTAL) != hipSuccess || hipMalloc(&R->bias, sizeof(float) * 48) ||
         hipMalloc(&R->a1, sizeof(bf16) * 16 * N3) ||
         hipMalloc(&R->a2, sizeof(bf16) * 16 * N3) || hipMalloc(&R->u, sizeof(float) * 2 * N3) ||
-        hipMalloc(&R->w, sizeof(float) * 2 * N3) ||
         hipMalloc(&R->gate, sizeof(int)) ||
         hipMalloc(&R->fallback, sizeof(long long)) || hipMalloc(&R->rmax, sizeof(double)))
         rc = fail(GABE_ERR_HIP, "closure attachment allocation failed");
@@ -1101,9 +1147,6 @@ namespace gabe_detail {
 int closure_after_step(gabe_ctx *c)
 {
     ClosureRun *R = c->closure;
-    hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, c->stream, c->sc, c->hist, c->hist_cap, R->thr, R->gate,
-                       R->fallback, R->rmax);
-    HIPCHK(hipGetLastError());
     const float *f = (const float *)c->fbuf[c->cur], *v = (const float *)c->vbuf[c->cur];
     ConvArgs a = conv_base(R->N, nullptr, R->pk + PK_F1, R->bias);
     a.fin[0] = f;
@@ -1111,6 +1154,7 @@ int closure_after_step(gabe_ctx *c)
     a.fin[2] = v;
     a.fin[3] = v + c->N3;
     for (int q = 0; q < 4; q++) a.fscale[q] = R->inv_s[q];
+    a.gatev = GateArgs{c->sc, c->hist, c->hist_cap, R->thr, R->gate, R->fallback, R->rmax};
     a.gate = R->gate;
     a.a_out = R->a1;
     CHK((launch_conv<4, EPI_HIDDEN>(a, c->stream)));
@@ -1123,10 +1167,10 @@ int closure_after_step(gabe_ctx *c)
     a.y_out = R->u;
     CHK((launch_conv<16, EPI_OUT>(a, c->stream)));
     float *vw = (float *)c->vbuf[c->cur];
-    hipLaunchKernelGGL(k_lap_out, dim3(nblocks(c->N3, 256)), dim3(256), 0, c->stream, R->u, c->N, R->w);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_lap_apply, dim3(nblocks(c->N3, 256)), dim3(256), 0, c->stream, R->w, c->N, vw, vw + c->N3,
-                       c->sc, R->dt_s[0], R->dt_s[1], R->gate);
+    const int N = c->N;
+    static_assert(KJ * KK == 256, "one site of each plane of a block per thread");
+    hipLaunchKernelGGL(k_lap2_kick, dim3((N / KK) * (N / KJ) * (N / KI)), dim3(256), 0, c->stream, R->u, N, vw,
+                       vw + c->N3, c->sc, R->dt_s[0], R->dt_s[1], R->gate);
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
@@ -1135,7 +1179,7 @@ void closure_release(gabe_ctx *c)
 {
     ClosureRun *R = c->closure;
     if (!R) return;
-    void *bufs[] = {R->pk, R->bias, R->a1, R->a2, R->u, R->w, R->gate, R->fallback, R->rmax};
+    void *bufs[] = {R->pk, R->bias, R->a1, R->a2, R->u, R->gate, R->fallback, R->rmax};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete R;
