@@ -439,7 +439,8 @@ __device__ __forceinline__ float2 lap2(const float *u, long s, int N, long N2)
  * w = lap u, in one pass: a workgroup owns a KI x KJ x KK block of sites,
  * computes w on the block and its 1-site halo into LDS (all of the u reads
  * of the block in flight at once), then corrects the block from LDS.  Both
- * Laplacians are lap2's arithmetic, site by site. */
+ * Laplacians are lap2's arithmetic, site by site.  Blocks go to the XCDs in
+ * contiguous ranges, so the halo reads of neighbouring blocks meet in one L2. */
 constexpr int KI = 4, KJ = 8, KK = 32, HJ = KJ + 2, HK = KK + 2, HP = HJ * HK, KH = (KI + 2) * HP;
 
 __device__ __forceinline__ float2 lap2_tile(const float2 *w, int h)
@@ -459,7 +460,7 @@ __global__ __launch_bounds__(256) void k_lap2_kick(const float *u, int N, float 
     if (gate && gate[0] == 0) return;
     __shared__ float2 w[KH];
     const long N2 = (long)N * N;
-    const int tid = threadIdx.x, ntk = N / KK, ntj = N / KJ, wg = blockIdx.x;
+    const int tid = threadIdx.x, ntk = N / KK, ntj = N / KJ, wg = logical_wg();
     const int k0 = (wg % ntk) * KK, j0 = ((wg / ntk) % ntj) * KJ, i0 = (wg / (ntk * ntj)) * KI;
 #pragma unroll 4
     for (int h = tid; h < KH; h += 256) {

@@ -5,6 +5,7 @@ their physical amplitude (no enlarged fluctuations):
 * 256^3 with tensor modes, 2 steps, FP64 and FP32 forms: pins the kernels
   the throughput figure comes from (k rows transformed in sweep A with the
   one-row split, the 256-wide j transform, the fused i transform + modes);
+* 512^3 with tensor modes (config 3's lattice), 2 steps, both forms;
 * 64^3 FP64 without tensor modes over 200 steps (the plain CPU workload),
   with the FP32 form beside it;
 * 256^3 FP32 against FP64 over the early window (200 steps);
@@ -96,6 +97,44 @@ def test_256_two_steps_match_oracle():
     ef, ed, eh, el, got = results["fp32"]
     assert ef < 1e-5 and ed < 1e-5 and eh < 1e-4 and el < 1e-4
     check_scalars(got, sc, rtol_a=1e-12, rtol_e=1e-10)
+
+
+def test_512_two_steps_match_oracle():
+    """Config 3's lattice: 512^3 with tensor modes, 2 steps from the seeded
+    vacuum state, FP64 and FP32 forms against 2 oracle steps.  Pins the 512^3
+    kernels: the FP32 form's k rows on 512-thread workgroups and its 512-wide
+    mode tiles, the FP64 form's library rows and separate i pass.  Tolerances
+    as at 256^3.  (About 90 s, most of it the oracle; its state at 512^3 with
+    tensor modes holds about 60 GB of host memory.)"""
+    cfg = configs.lambda_phi4_config(N=512, calc_gws=1)
+    o = seeded_oracle(cfg, nthreads=THREADS)
+    lats = {name: lattice_from_oracle(o, cfg, prec) for name, prec in (("fp64", GABE_FP64), ("fp32", GABE_FP32))}
+    o.step(2)
+    sc = o.scalars()
+    ref_sp = o.specout(gw_quirk=1)
+    for name, lat in lats.items():
+        lat.step(2)
+        sp = lat.specOut()
+        report_nonfinite(name, sp)
+        for col in range(ref_sp.shape[0]):
+            live = live_bins(ref_sp[col])
+            err = float(np.max(np.abs(sp[col][live] / ref_sp[col][live] - 1)))
+            print(f"{name} spectrum column {col}: worst bin {err:.2e}", flush=True)
+            assert err < (1e-8 if name == "fp64" else 1e-3), (name, col)
+        f, d = lat.download_state()
+        ef, ed = fluct_err(f, o.field[0]), fluct_err(d, o.dfield[0])
+        del f, d
+        h, l = lat.download_gw()
+        eh, el = rel_err(h, o.h[0]), rel_err(l, o.l[0])
+        del h, l
+        print(f"{name}: field {ef:.2e} velocity {ed:.2e} h {eh:.2e} l {el:.2e}", flush=True)
+        if name == "fp64":
+            assert ef < 1e-9 and ed < 1e-9 and eh < 1e-9 and el < 1e-9
+            check_scalars(lat.get_scalars(), sc)
+        else:
+            assert ef < 1e-5 and ed < 1e-5 and eh < 1e-4 and el < 1e-4
+            check_scalars(lat.get_scalars(), sc, rtol_a=1e-12, rtol_e=1e-10)
+        lat.close()
 
 
 def test_64_fp64_200_steps_match_oracle():
