@@ -86,7 +86,8 @@ def run_device(cfg, precision, steps, warmup, profile_steps, ppb=0):
     from gabe_amd import host
     lat = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb)
     q = lat.query()
-    tiling = {"planes_per_block": q.planes_per_block, "transform_tile": q.transform_tile, "mode_tile": q.mode_tile}
+    tiling = {"planes_per_block": q.planes_per_block, "transform_tile": q.transform_tile, "mode_tile": q.mode_tile,
+              "stress_chunk": q.stress_chunk}
     lat.step(warmup)
     lat.sync()
     t0 = time.perf_counter()

@@ -225,6 +225,43 @@ int stress_fft(gabe_ctx *c, bool with_i = true, hipStream_t st = nullptr)
     return GABE_OK;
 }
 
+/* sweep A in launches of c->chunk plane blocks on the main stream, each
+ * followed on jst by the j-transform of its planes (kz = N/2 split off), so
+ * that the spectra a launch wrote are transformed while still in the
+ * memory-side cache.  ev_jdone marks the last transform. */
+int chunked_sweep_a(gabe_ctx *c)
+{
+    const int gy = (int)c->grid.y, bpc = c->chunk;
+    const long N = c->N, kp = c->kpad;
+    const size_t csz = 2 * c->esz;   /* bytes per complex element */
+    char *nyq = (char *)nyq_plane(c);
+    for (int q = 0, b0 = 0; b0 < gy; q++, b0 += bpc) {
+        const int nb = std::min(bpc, gy - b0);
+        const dim3 grid(c->grid.x, (unsigned)nb);
+        if (c->fp32) {
+            SweepArgs<float> a = sweep_args<float>(c);
+            a.by0 = b0;
+            launch_sweep<float>(SW_A, c->nf, true, a, grid, c->stream);
+        } else {
+            SweepArgs<double> a = sweep_args<double>(c);
+            a.by0 = b0;
+            launch_sweep<double>(SW_A, c->nf, true, a, grid, c->stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev_chunk[q], c->stream));
+        HIPCHK(hipStreamWaitEvent(c->jst, c->ev_chunk[q], 0));
+        const long p0 = (long)b0 * c->ci, np = std::min((long)nb * c->ci, N - p0);
+        /* [comp][plane p0 + s][j][kz]: outer = component, sub = plane */
+        ColFftView vj{(char *)c->shat + csz * p0 * N * kp, kp, N * N * kp, N * kp, (int)(N / 2), (int)np,
+                      GABE_NSC, nyq + csz * p0 * N, N * N};
+        if (c->fp32) launch_colfft<float>(vj, c->N, c->fft_cw, c->twiddle, c->jst);
+        else launch_colfft<double>(vj, c->N, c->fft_cw, c->twiddle, c->jst);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(c->ev_jdone, c->jst));
+    return GABE_OK;
+}
+
 /* tensor modes of one step: i transform + TT projection + both RK2 stages */
 int gw_modes(gabe_ctx *c, hipStream_t st = nullptr)
 {
@@ -262,7 +299,8 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
      * the step two back, which read them, is done */
     if (pipe) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_gw[c->slot], 0));
     CHK(rec(0));
-    CHK(launch_sweep_ctx(c, SW_A, c->gw));
+    if (pipe && c->chunk) CHK(chunked_sweep_a(c));
+    else CHK(launch_sweep_ctx(c, SW_A, c->gw));
     CHK(rec(1));
     CHK(rec(2));
     CHK(launch_finalize_ctx(c, SW_A, 0));
@@ -270,7 +308,8 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
     if (pipe) {
         HIPCHK(hipEventRecord(c->ev_fin, c->stream));
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fin, 0));
-        CHK(stress_fft(c, false, c->side));
+        if (c->chunk) HIPCHK(hipStreamWaitEvent(c->side, c->ev_jdone, 0));
+        else CHK(stress_fft(c, false, c->side));
         CHK(gw_modes(c, c->side));
         HIPCHK(hipEventRecord(c->ev_gw[c->slot], c->side));
         CHK(launch_sweep_ctx(c, SW_B, false));
@@ -339,6 +378,7 @@ struct Tiling {
     int planes = 0;      /* sweep planes per workgroup */
     int cw = 0;          /* j-transform columns per workgroup */
     int mode_tile = 0;   /* fused i-transform + mode kernel variant */
+    int chunk = 0;       /* planes per interleaved sweep-A / j-transform launch (-1: none) */
 };
 std::map<long, Tiling> g_tuned;
 std::mutex g_tuned_mu;
@@ -363,6 +403,7 @@ int check_params(const gabe_params *p)
     if ((long)p->N * p->N * p->N > (1L << 31)) return fail(GABE_ERR_ARG, "N^3 beyond 2^31 is not supported");
     if (p->transform_tile != 0 && p->transform_tile != 16 && p->transform_tile != 32)
         return fail(GABE_ERR_ARG, "transform_tile must be 0, 16 or 32 (got %d)", p->transform_tile);
+    if (p->stress_chunk < -1) return fail(GABE_ERR_ARG, "stress_chunk must be -1, 0 or a plane count (got %d)", p->stress_chunk);
     const int nt = gabe_gwfft_tiles(p->N, p->precision == GABE_FP32);
     if (p->mode_tile < 0 || p->mode_tile > std::max(nt, 1))
         return fail(GABE_ERR_ARG, "mode_tile must be 0..%d at N=%d (got %d)", std::max(nt, 1), p->N, p->mode_tile);
@@ -458,6 +499,18 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->ci = ci;
     c->grid = dim3(bx, by);
     c->nblk = bx * by;
+    if (c->pipe && c->gw) {   /* interleaved sweep A / j-transform launches */
+        int planes = params->stress_chunk;
+        if (planes == 0) planes = tuned_tiling(c->N, c->nf, c->fp32, c->alloc_gw).chunk;
+        const int bpc = planes > 0 ? planes / ci : 0;
+        if (bpc >= 1 && bpc < by) {
+            c->chunk = bpc;
+            HIPCHK(hipStreamCreateWithFlags(&c->jst, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&c->ev_jdone, hipEventDisableTiming));
+            c->ev_chunk.assign((size_t)((by + bpc - 1) / bpc), nullptr);
+            for (hipEvent_t &e : c->ev_chunk) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+    }
     HIPCHK(hipMalloc(&c->partials, sizeof(double) * GABE_MAXSUMS * std::max(c->nblk, 4096)));
 
     HIPCHK(hipMalloc(&c->sc, sizeof(DevScalars)));
@@ -517,6 +570,11 @@ void destroy_impl(gabe_ctx *c)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->jst) (void)hipStreamSynchronize(c->jst);
+    for (hipEvent_t e : c->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
+    if (c->ev_jdone) (void)hipEventDestroy(c->ev_jdone);
+    if (c->jst) (void)hipStreamDestroy(c->jst);
     for (hipEvent_t e : c->ev_gw)
         if (e) (void)hipEventDestroy(e);
     if (c->ev_fin) (void)hipEventDestroy(c->ev_fin);
@@ -1297,6 +1355,7 @@ int gabe_query(gabe_ctx *c, gabe_params *out)
     out->planes_per_block = c->ci;
     out->transform_tile = c->colfft ? c->fft_cw : 0;
     out->mode_tile = c->gwfft ? c->gw_tile : 0;
+    out->stress_chunk = c->chunk * c->ci;
     return GABE_OK;
 }
 
@@ -1314,6 +1373,7 @@ int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double
         p.planes_per_block = t.planes;
         p.transform_tile = t.cw;
         p.mode_tile = t.mode_tile;
+        p.stress_chunk = t.chunk ? t.chunk : -1;
         gabe_ctx *c = new gabe_ctx;
         int rc = create_impl(&p, c);
         if (rc == GABE_OK) {
@@ -1361,6 +1421,19 @@ int gabe_autotune_sweep(const gabe_params *params, int *planes_per_block, double
                     best_ms = ms;
                 }
             }
+        /* sweep A and the j-transform interleaved in launches of N/8, N/4 or
+         * N/2 planes (where that is a whole number of at least one block) */
+        best.chunk = -1;
+        for (int planes : {p.N / 8, p.N / 4, p.N / 2}) {
+            if (planes < best.planes || planes % best.planes) continue;
+            float ms = 0.f;
+            const Tiling t{best.planes, best.cw, best.mode_tile, planes};
+            CHK(trial(t, ms, fused));
+            if (ms < best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        }
     }
     {
         std::lock_guard<std::mutex> lock(g_tuned_mu);

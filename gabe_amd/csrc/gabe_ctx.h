@@ -4,6 +4,8 @@
  */
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <rocfft/rocfft.h>
 
 #include <string>
@@ -70,6 +72,15 @@ struct gabe_ctx {
     void *shat_b[2] = {nullptr, nullptr};
     StepScalars *ss_b[2] = {nullptr, nullptr};
     hipEvent_t ev_gw[2] = {nullptr, nullptr};
+    /* chunked stress transform (pipelined steps, chunk > 0): sweep A runs in
+     * launches of chunk plane blocks and the j-transform of each launch's
+     * planes follows on jst as soon as it is done, while the spectra it reads
+     * are still in the memory-side cache; ev_chunk[q] marks sweep A's launch q
+     * done, ev_jdone the last j-transform */
+    int chunk = 0;
+    hipStream_t jst = nullptr;
+    std::vector<hipEvent_t> ev_chunk;
+    hipEvent_t ev_jdone = nullptr;
     void *fbuf[2] = {nullptr, nullptr};  /* ping-pong state, [nf][N3] each */
     void *vbuf[2] = {nullptr, nullptr};
     int cur = 0;

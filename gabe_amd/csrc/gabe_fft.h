@@ -62,17 +62,19 @@ bool gabe_colfft_supported(int N);
  * [outer][N][nsub][cols] at o*outer_stride + r*axis_stride + s*sub_stride + c
  * (complex elements), transformed along r for every o, s and c < sub_valid.
  *
- * With nyq set (nsub = 1), column 0 holds two real sequences packed as
+ * With nyq set, column 0 of every (o, s) holds two real sequences packed as
  * x + i y (the kz = 0 and kz = N/2 values of real-input rows).  Their
  * transforms X, Y are Hermitian in the transform index n, so
  *   X(n) = (Z(n) + conj Z(N-n)) / 2,   Y(n) = (Z(n) - conj Z(N-n)) / 2i;
- * X goes back into column 0 and Y to nyq[o*N + n]. */
+ * X goes back into column 0 and Y to nyq[o*nyq_outer + s*N + n]
+ * (nyq_outer 0 means N, for views with nsub = 1). */
 struct ColFftView {
     void *data;
     long axis_stride, outer_stride, sub_stride;
     int sub_valid, nsub;
     long outer;
     void *nyq = nullptr;
+    long nyq_outer = 0;
 };
 
 /* In-place forward complex transform of length N along r.  cw is the column

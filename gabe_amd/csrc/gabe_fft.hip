@@ -36,7 +36,7 @@ template <typename T, int N1, int N2, int CW>
 __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type *data, long axis_stride,
                                                        long outer_stride, long sub_stride, int sub_valid,
                                                        int chunks_per_sub, const typename cplx_t<T>::type *tw,
-                                                       typename cplx_t<T>::type *nyq)
+                                                       typename cplx_t<T>::type *nyq, long nyq_outer)
 {
     using C = typename cplx_t<T>::type;
     constexpr int N = N1 * N2;
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
     if (gridDim.x % 8 == 0) b = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
     const int sidx = b / chunks_per_sub, q = b % chunks_per_sub;
     const int width = min(CW, sub_valid - q * CW);
-    const bool split = nyq != nullptr && sidx == 0 && q == 0;   /* uniform over the workgroup */
+    const bool split = nyq != nullptr && q == 0;   /* uniform over the workgroup */
     C *base = data + (long)blockIdx.y * outer_stride + (long)sidx * sub_stride + (long)q * CW;
 
     /* first step: task (n2, cc) loads x[n1] = row N2 n1 + n2, transforms it
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kColThreads) void k_colfft(typename cplx_t<T>::type
     }
     if (split) {
         __syncthreads();
-        C *ny = nyq + (long)blockIdx.y * N;
+        C *ny = nyq + (long)blockIdx.y * nyq_outer + (long)sidx * N;
         const T h = (T)0.5;
         for (int n = threadIdx.x; n < N; n += kColThreads) {
             const C Z = packed[n], W = packed[(N - n) & (N - 1)];
@@ -130,7 +130,8 @@ void launch_cw(const ColFftView &v, const void *tw, hipStream_t s)
     const int chunks = (v.sub_valid + CW - 1) / CW;
     const dim3 grid((unsigned)(v.nsub * chunks), (unsigned)v.outer);
     hipLaunchKernelGGL((k_colfft<T, N1, N2, CW>), grid, dim3(kColThreads), 0, s, (C *)v.data, v.axis_stride,
-                       v.outer_stride, v.sub_stride, v.sub_valid, chunks, (const C *)tw, (C *)v.nyq);
+                       v.outer_stride, v.sub_stride, v.sub_valid, chunks, (const C *)tw, (C *)v.nyq,
+                       v.nyq_outer ? v.nyq_outer : (long)N1 * N2);
 }
 
 template <typename T, int N1, int N2>

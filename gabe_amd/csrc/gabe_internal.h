@@ -76,6 +76,8 @@ struct SweepArgs {
     gabe_model model;
     int N, ci, nblk;
     int gx, gy;             /* workgroup grid of this lattice's sweeps */
+    int by0;                /* first plane block of the launch: a launch of gridDim.y
+                             * blocks covers plane blocks by0 .. by0 + gridDim.y - 1 */
     long N2, N3;
     double dt, inv_dx2;
 };

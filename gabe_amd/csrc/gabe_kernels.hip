@@ -248,6 +248,7 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
             by = logical / gx;
         }
     }
+    by += A.by0;
     const long p = (long)bx * GABE_BLOCK + threadIdx.x;
     const bool valid = p < N2;
     const long pp = valid ? p : 0;
@@ -381,6 +382,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx
             by = logical / gx;
         }
     }
+    by += A.by0;
     constexpr int BLK = ROWN > 0 ? row_block(ROWN) : GABE_BLOCK;
     const long p = (long)bx * BLK + threadIdx.x;
     const bool valid = p < N2;

@@ -47,7 +47,7 @@ class GabeParams(C.Structure):
         ("calc_gws", C.c_int), ("expansion_type", C.c_int), ("precision", C.c_int),
         ("m2_phi", C.c_double), ("lam", C.c_double), ("g2", C.c_double), ("m2_chi", C.c_double),
         ("gw_spectrum_quirk", C.c_int), ("planes_per_block", C.c_int), ("transform_tile", C.c_int),
-        ("mode_tile", C.c_int), ("reserved", C.c_int * 4),
+        ("mode_tile", C.c_int), ("stress_chunk", C.c_int), ("reserved", C.c_int * 3),
     ]
 
 
@@ -155,11 +155,11 @@ def numbins(N):
 
 def make_params(N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1, precision=GABE_FP64, m2_phi=1.0,
                 lam=0.0, g2=0.0, m2_chi=0.0, gw_spectrum_quirk=1, planes_per_block=0, transform_tile=0,
-                mode_tile=0):
+                mode_tile=0, stress_chunk=0):
     if dt is None:
         dt = L / N / 20.0
     return GabeParams(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
-                      gw_spectrum_quirk, planes_per_block, transform_tile, mode_tile)
+                      gw_spectrum_quirk, planes_per_block, transform_tile, mode_tile, stress_chunk)
 
 
 def means_vars_noalloc(dev_ptr, precision, nsites, nflds, base=None, stream=None):
@@ -199,9 +199,10 @@ class Lattice:
 
     def __init__(self, N, nflds=2, L=10.0, dt=None, calc_gws=0, expansion_type=1,
                  precision=GABE_FP64, m2_phi=1.0, lam=0.0, g2=0.0, m2_chi=0.0,
-                 gw_spectrum_quirk=1, planes_per_block=0, transform_tile=0, mode_tile=0, _handle=None):
+                 gw_spectrum_quirk=1, planes_per_block=0, transform_tile=0, mode_tile=0, stress_chunk=0,
+                 _handle=None):
         self.params = make_params(N, nflds, L, dt, calc_gws, expansion_type, precision, m2_phi, lam, g2, m2_chi,
-                                  gw_spectrum_quirk, planes_per_block, transform_tile, mode_tile)
+                                  gw_spectrum_quirk, planes_per_block, transform_tile, mode_tile, stress_chunk)
         self.N, self.nflds, self.calc_gws = N, nflds, calc_gws
         self.M = N * N * (N // 2 + 1)
         self._owned = _handle is None

@@ -69,7 +69,14 @@ typedef struct gabe_params {
     int mode_tile;         /* tile variant of the fused i-transform + mode kernel,
                               1..gabe_mode_tiles(N, precision) (0 = autotuned or
                               variant 1, the built-in default)                     */
-    int reserved[4];
+    int stress_chunk;      /* planes per launch when sweep A and the j-transform
+                              of the stress spectra are interleaved (pipelined
+                              FP32 steps with tensor modes; rounded down to whole
+                              planes_per_block): each launch's spectra are
+                              transformed along j while still in the memory-side
+                              cache.  0 = autotuned or not interleaved, -1 = not
+                              interleaved                                          */
+    int reserved[3];
 } gabe_params;
 
 /* The reference's expansion/energy globals (g2header.h:59-64) plus bookkeeping. */
@@ -211,7 +218,8 @@ int gabe_query(gabe_ctx *ctx, gabe_params *out);
 /* Tiling autotuner: times a few steps of a synthetic lattice of these
  * parameters for each candidate planes_per_block; with tensor modes on the
  * fused transform path it then times, at the best planes_per_block, every
- * j-transform tile width and mode-kernel tile variant.  The fastest choice is
+ * j-transform tile width and mode-kernel tile variant, and then the
+ * stress_chunk candidates (not interleaved, N/8, N/4, N/2).  The fastest choice is
  * remembered for (N, nflds, precision, calc_gws); later contexts created with
  * 0 in those fields use it (gabe_query reports what a context uses).  Either
  * output pointer may be NULL; ms_per_step is the best step time.  No

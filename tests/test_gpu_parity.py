@@ -417,6 +417,37 @@ def test_transform_and_mode_tiles_agree_at_256(precision):
         lat.close()
 
 
+@pytest.mark.parametrize("N,precision", [(256, GABE_FP32), (256, GABE_FP64), (128, GABE_FP32), (512, GABE_FP32)])
+def test_interleaved_stress_chunks_are_bitwise_equal(N, precision):
+    """Sweep A and the j-transform interleaved in launches of a few plane
+    blocks (each launch's spectra transformed along j on a stream of their
+    own) run the same kernels on the same values as the whole-lattice
+    launches: fields, velocities, h, l and the scalars after 3 steps are
+    bitwise equal, for every chunk size the autotuner tries."""
+    from gabe_amd import host
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=1)
+    ppb = N // 16
+    ref = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb, stress_chunk=-1)
+    assert ref.query().stress_chunk == 0
+    ref.step(3)
+    f0, v0 = ref.download_state()
+    h0, l0 = ref.download_gw()
+    s0 = ref.get_scalars()
+    ref.close()
+    for planes in (N // 8, N // 4, N // 2):
+        lat = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb, stress_chunk=planes)
+        assert lat.query().stress_chunk == planes
+        lat.step(3)
+        f, v = lat.download_state()
+        h, l = lat.download_gw()
+        s = lat.get_scalars()
+        lat.close()
+        print(f"N={N} chunk {planes} planes: fields {np.array_equal(f, f0)}, h/l {np.array_equal(h, h0)}", flush=True)
+        assert np.array_equal(f, f0) and np.array_equal(v, v0), planes
+        assert np.array_equal(h, h0) and np.array_equal(l, l0), planes
+        assert s == s0, planes
+
+
 def test_tile_parameters_are_validated():
     cfg = configs.lambda_phi4_config(N=32, calc_gws=1)
     from gabe_amd.lattice import GabeError, Lattice
@@ -424,3 +455,5 @@ def test_tile_parameters_are_validated():
         Lattice(precision=GABE_FP32, transform_tile=8, **cfg.lattice_kwargs())
     with pytest.raises(GabeError):
         Lattice(precision=GABE_FP32, mode_tile=2, **cfg.lattice_kwargs())
+    with pytest.raises(GabeError):
+        Lattice(precision=GABE_FP32, stress_chunk=-2, **cfg.lattice_kwargs())
