@@ -58,3 +58,37 @@ def test_invalid_params_fail_loudly_without_gpu_work():
     assert e.value.code == lattice.GABE_ERR_ARG
     with pytest.raises(lattice.GabeError):
         lattice.Lattice(N=16, nflds=3)
+    with pytest.raises(lattice.GabeError) as e:
+        lattice.Lattice(N=16, stress_chunk=-2)
+    assert e.value.code == lattice.GABE_ERR_ARG
+
+
+def test_ctypes_mirror_matches_the_header_layout(tmp_path):
+    """gabe_params and gabe_scalars as the C compiler lays them out from
+    include/gabe.h: size and every field's offset equal the ctypes mirror's."""
+    import subprocess
+    from gabe_amd import lattice
+    structs = {"gabe_params": lattice.GabeParams, "gabe_scalars": lattice.GabeScalars}
+    # the mirror names two fields differently from the header
+    cname = {"lam": "lambda"}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gabe.h"', "int main(void) {"]
+    for st, py in structs.items():
+        lines.append(f'printf("{st} size %zu\\n", sizeof({st}));')
+        for f, _ in py._fields_:
+            c = cname.get(f, f)
+            lines.append(f'printf("{st} {f} %zu\\n", offsetof({st}, {c}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            st, f, v = line.split()
+            got[(st, f)] = int(v)
+    for st, py in structs.items():
+        assert got[(st, "size")] == ctypes.sizeof(py), st
+        for f, _ in py._fields_:
+            assert got[(st, f)] == getattr(py, f).offset, (st, f)
