@@ -190,7 +190,7 @@ def run_ensemble(steps, warmup):
         "sequential_ms_per_step": 1e3 * t_seq / steps,
         "kernels_ms": ktimes,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"ensemble_{dom}", "64lat"),
                      "alg_bytes_per_launch": alg[dom] * sites},
         "fields_frac": 48.0 * rate / (HBM_PEAK_GBS * 1e9),
         "planes_per_block": tiling,

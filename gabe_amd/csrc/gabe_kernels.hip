@@ -231,23 +231,34 @@ __device__ __forceinline__ void row_r2c(typename gabe_fft::cplx_t<T>::type *zl, 
  * gains nothing: the wait for the next plane's neighbour loads also waits
  * for every load issued before them.)  The arithmetic is sweep_body's,
  * expression for expression. */
+/* XCD-aware block order over the nhw hardware workgroups a lattice has in
+ * its launch (nhw >= gx gy).  Consecutive hardware workgroups go round-robin
+ * to the 8 XCDs, each with its own L2; every XCD gets a contiguous range of
+ * logical blocks instead, so the j+-1 neighbour rows a workgroup reads are
+ * being read by workgroups on the same XCD at the same time.  With nhw a
+ * multiple of 8 the map is a bijection of [0, nhw) and logical blocks past
+ * gx gy are padding (a batched launch pads every lattice's range to a
+ * multiple of 8); otherwise the order is the hardware's.  Returns false for
+ * a padding workgroup. */
+__device__ __forceinline__ bool xcd_block(int hw, int gx, int gy, int nhw, int &bx, int &by)
+{
+    int logical = hw;
+    if (nhw % GABE_NXCD == 0) logical = (hw % GABE_NXCD) * (nhw / GABE_NXCD) + hw / GABE_NXCD;
+    bx = logical % gx;
+    by = logical / gx;
+    return logical < gx * gy;
+}
+
 template <int Q>
 using plane_role = std::integral_constant<int, Q>;
 
 template <typename T, int NF>
-__device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int gx, int gy)
+__device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int gx, int gy, int nhw)
 {
     const int N = A.N;
     const long N2 = A.N2;
-    int bx = hw % gx, by = hw / gx;
-    {
-        const int nb = gx * gy;
-        if (nb % GABE_NXCD == 0) {
-            const int logical = (hw % GABE_NXCD) * (nb / GABE_NXCD) + hw / GABE_NXCD;
-            bx = logical % gx;
-            by = logical / gx;
-        }
-    }
+    int bx, by;
+    if (!xcd_block(hw, gx, gy, nhw, bx, by)) return;
     by += A.by0;
     const long p = (long)bx * GABE_BLOCK + threadIdx.x;
     const bool valid = p < N2;
@@ -357,31 +368,20 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
 }
 
 template <typename T, int NF, int MODE, bool GW, int ROWN = 0>
-__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy,
+__device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy, int nhw,
                                            typename gabe_fft::cplx_t<T>::type *zl = nullptr,
                                            typename gabe_fft::cplx_t<T>::type *zo = nullptr,
                                            typename gabe_fft::cplx_t<T>::type *zt = nullptr)
 {
     if constexpr (MODE == SW_B && sizeof(T) == 4) {
-        sweep_b_ahead<T, NF>(A, hw, gx, gy);
+        sweep_b_ahead<T, NF>(A, hw, gx, gy, nhw);
         return;
     }
     constexpr int NS = gabe_nsums(NF);
     const int N = A.N;
     const long N2 = A.N2;
-    /* XCD-aware block order: consecutive hardware workgroups go round-robin to
-     * the 8 XCDs, each with its own L2.  Give every XCD a contiguous range of
-     * rows instead, so the j+-1 neighbour rows a workgroup reads are being
-     * read by workgroups on the same XCD at the same time. */
-    int bx = hw % gx, by = hw / gx;
-    {
-        const int nb = gx * gy;
-        if (nb % GABE_NXCD == 0) {
-            const int logical = (hw % GABE_NXCD) * (nb / GABE_NXCD) + hw / GABE_NXCD;
-            bx = logical % gx;
-            by = logical / gx;
-        }
-    }
+    int bx, by;
+    if (!xcd_block(hw, gx, gy, nhw, bx, by)) return;
     by += A.by0;
     constexpr int BLK = ROWN > 0 ? row_block(ROWN) : GABE_BLOCK;
     const long p = (long)bx * BLK + threadIdx.x;
@@ -602,13 +602,14 @@ __global__ __launch_bounds__(row_block(ROWN)) void k_sweep_rowfft(SweepArgs<T> A
      * so that three workgroups fit in the LDS of one CU (a whole table, 2 KB
      * more, measured at two) */
     __shared__ C zl[rbuf_size<T>(ROWN)], zo[rbuf_size<T>(ROWN)], zt[ROWN / 2];
-    sweep_body<T, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, zl, zo, zt);
+    sweep_body<T, NF, SW_A, true, ROWN>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y,
+                                        gridDim.x * gridDim.y, zl, zo, zt);
 }
 
 template <typename T, int NF, int MODE, bool GW>
 __global__ __launch_bounds__(GABE_BLOCK) void k_sweep(SweepArgs<T> A)
 {
-    sweep_body<T, NF, MODE, GW>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y);
+    sweep_body<T, NF, MODE, GW>(A, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, gridDim.x * gridDim.y);
 }
 
 /* every lattice of an ensemble in one launch: workgroups [start[L], start[L+1])
@@ -628,9 +629,9 @@ __global__ __launch_bounds__(GABE_BLOCK) void k_sweep_batch(const SweepArgs<T> *
      * that the compiler reloads the descriptor's fields after every store of
      * the march (batched sweep B at 64 lattices: 637 -> 490 us) */
     const SweepArgs<T> &A = descs[lo];
-    const int hw = b - start[lo];
-    if (hw >= A.gx * A.gy) return;   /* padding workgroup */
-    sweep_body<T, NF, MODE, GW>(A, hw, A.gx, A.gy);
+    /* the lattice's range is padded to a multiple of 8 workgroups: the block
+     * order spreads the padding over the XCDs' ranges */
+    sweep_body<T, NF, MODE, GW>(A, b - start[lo], A.gx, A.gy, start[lo + 1] - start[lo]);
 }
 
 /* ---------------------------------------------------------------- finalize */
