@@ -134,6 +134,18 @@ def detach(lat):
     _check(load().gabe_closure_detach(lat._h))
 
 
+CLOSURE_KERNELS = ("first", "hidden", "output", "kick")
+
+
+def kernel_times(lat, nsteps):
+    """mean ms per step of the attached closure's kernels (first, hidden and
+    output convolutions, the bi-Laplacian kick), timed with HIP events over
+    nsteps steps launched one kernel after another"""
+    out = (C.c_double * len(CLOSURE_KERNELS))()
+    _check(load().gabe_closure_kernel_times(lat._h, nsteps, out))
+    return dict(zip(CLOSURE_KERNELS, list(out)))
+
+
 def status(lat):
     """(active, fallback_step, max_residual)"""
     a, f, r = C.c_int(), C.c_longlong(), C.c_double()

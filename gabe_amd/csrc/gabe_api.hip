@@ -288,7 +288,7 @@ int gw_modes(gabe_ctx *c, hipStream_t st = nullptr)
  * concurrently with the stress transform and the GW modes; the step ends when
  * both are done.  With ev, the kernels run one after another so that each
  * class is timed alone. */
-int one_step(gabe_ctx *c, hipEvent_t *ev)
+int one_step(gabe_ctx *c, hipEvent_t *ev, hipEvent_t *cl_ev = nullptr)
 {
     auto rec = [&](int k) -> int {
         if (ev) HIPCHK(hipEventRecord(ev[k], c->stream));
@@ -317,7 +317,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
         c->slot ^= 1;
         c->shat = c->shat_b[c->slot];
         c->ss = c->ss_b[c->slot];
-        if (c->closure) CHK(closure_after_step(c));
+        if (c->closure) CHK(closure_after_step(c, cl_ev));
         return GABE_OK;
     }
     const bool overlap = c->gw && !ev && c->side;
@@ -343,7 +343,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev)
         c->shat = c->shat_b[c->slot];
         c->ss = c->ss_b[c->slot];
     }
-    if (c->closure) CHK(closure_after_step(c));
+    if (c->closure) CHK(closure_after_step(c, cl_ev));
     return GABE_OK;
 }
 
@@ -1218,6 +1218,36 @@ int gabe_kernel_times(gabe_ctx *c, int nsteps, double *ms_out)
             }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int gabe_closure_kernel_times(gabe_ctx *c, int nsteps, double *ms_out)
+{
+    if (!c || !ms_out || nsteps <= 0) return fail(GABE_ERR_ARG, "bad argument");
+    if (!c->closure) return fail(GABE_ERR_STATE, "no closure is attached to this lattice");
+    const int nev = 2 * GABE_NKCLASS, ncl = 2 * GABE_NCLCLASS;
+    std::vector<hipEvent_t> ev((size_t)nev * nsteps), cev((size_t)ncl * nsteps);
+    int rc = GABE_OK;
+    for (auto &e : ev)
+        if (rc == GABE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(GABE_ERR_HIP, "event creation failed");
+    for (auto &e : cev)
+        if (rc == GABE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(GABE_ERR_HIP, "event creation failed");
+    for (int n = 0; n < nsteps && rc == GABE_OK; n++)
+        rc = one_step(c, ev.data() + (size_t)n * nev, cev.data() + (size_t)n * ncl);
+    if (rc == GABE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GABE_ERR_HIP, "sync failed");
+    for (int k = 0; k < GABE_NCLCLASS; k++) ms_out[k] = 0.;
+    if (rc == GABE_OK) {
+        for (int n = 0; n < nsteps; n++)
+            for (int k = 0; k < GABE_NCLCLASS; k++) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, cev[(size_t)n * ncl + 2 * k], cev[(size_t)n * ncl + 2 * k + 1]);
+                ms_out[k] += ms / nsteps;
+            }
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : cev)
+        if (e) (void)hipEventDestroy(e);
     return rc;
 }
 

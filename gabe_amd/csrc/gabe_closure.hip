@@ -1145,9 +1145,13 @@ int run_create(gabe_ctx *c, const gabe_detail::ClosureCkpt &k, const void *pk, c
 
 namespace gabe_detail {
 
-int closure_after_step(gabe_ctx *c)
+int closure_after_step(gabe_ctx *c, hipEvent_t *ev)
 {
     ClosureRun *R = c->closure;
+    auto rec = [&](int k) -> int {
+        if (ev) HIPCHK(hipEventRecord(ev[k], c->stream));
+        return GABE_OK;
+    };
     const float *f = (const float *)c->fbuf[c->cur], *v = (const float *)c->vbuf[c->cur];
     ConvArgs a = conv_base(R->N, nullptr, R->pk + PK_F1, R->bias);
     a.fin[0] = f;
@@ -1158,21 +1162,29 @@ int closure_after_step(gabe_ctx *c)
     a.gatev = GateArgs{c->sc, c->hist, c->hist_cap, R->thr, R->gate, R->fallback, R->rmax};
     a.gate = R->gate;
     a.a_out = R->a1;
+    CHK(rec(0));
     CHK((launch_conv<4, EPI_HIDDEN>(a, c->stream)));
+    CHK(rec(1));
     a = conv_base(R->N, R->a1, R->pk + PK_F2, R->bias + 16);
     a.gate = R->gate;
     a.a_out = R->a2;
+    CHK(rec(2));
     CHK((launch_conv<16, EPI_HIDDEN>(a, c->stream)));
+    CHK(rec(3));
     a = conv_base(R->N, R->a2, R->pk + PK_F3, R->bias + 32);
     a.gate = R->gate;
     a.y_out = R->u;
+    CHK(rec(4));
     CHK((launch_conv<16, EPI_OUT>(a, c->stream)));
+    CHK(rec(5));
     float *vw = (float *)c->vbuf[c->cur];
     const int N = c->N;
     static_assert(KJ * KK == 256, "one site of each plane of a block per thread");
+    CHK(rec(6));
     hipLaunchKernelGGL(k_lap2_kick, dim3((N / KK) * (N / KJ) * (N / KI)), dim3(256), 0, c->stream, R->u, N, vw,
                        vw + c->N3, c->sc, R->dt_s[0], R->dt_s[1], R->gate);
     HIPCHK(hipGetLastError());
+    CHK(rec(7));
     return GABE_OK;
 }
 

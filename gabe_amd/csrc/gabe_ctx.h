@@ -125,7 +125,7 @@ void destroy_plan(FftPlan &P);
 int read_scalars(gabe_ctx *c, DevScalars &d);
 int write_scalars(gabe_ctx *c, const DevScalars &d);
 /* closure kernels after a step (gabe_closure.hip) and the attachment's teardown */
-int closure_after_step(gabe_ctx *c);
+int closure_after_step(gabe_ctx *c, hipEvent_t *ev = nullptr);   /* ev: 8 events, before/after each kernel */
 void closure_release(gabe_ctx *c);
 /* the attached closure's part of a checkpoint: its gate (on/off), the step it
  * switched off at, its largest residual, threshold, scales and weights */

@@ -38,6 +38,7 @@ EXPORTS = (
     "gabe_closure_set_params", "gabe_closure_scales", "gabe_closure_set_scales", "gabe_closure_sample",
     "gabe_closure_num_samples", "gabe_closure_train", "gabe_closure_loss", "gabe_closure_forward",
     "gabe_closure_gradient", "gabe_closure_attach", "gabe_closure_detach", "gabe_closure_status",
+    "gabe_closure_kernel_times",
 )
 
 
@@ -128,6 +129,7 @@ def load():
         "gabe_closure_attach": [vp, vp, C.c_double],
         "gabe_closure_detach": [vp],
         "gabe_closure_status": [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), dp],
+        "gabe_closure_kernel_times": [vp, ip, dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -329,6 +329,14 @@ int gabe_closure_detach(gabe_ctx *ctx);
 /* active: 1 while the closure is applied; fallback_step: step count when it
  * switched off (-1 if never); max_residual: largest R checked so far. */
 int gabe_closure_status(gabe_ctx *ctx, int *active, long long *fallback_step, double *max_residual);
+/* Timing hook for the attached closure (no reference counterpart): runs
+ * nsteps steps one kernel after another, as gabe_kernel_times does, with HIP
+ * events around each closure kernel on the lattice's stream, and writes the
+ * mean ms per step to ms_out[0..GABE_NCLCLASS): first convolution (4 -> 16
+ * channels), hidden convolution (16 -> 16), output convolution (16 -> 2),
+ * the bi-Laplacian kick.  GABE_ERR_STATE without an attached closure. */
+#define GABE_NCLCLASS 4
+int gabe_closure_kernel_times(gabe_ctx *ctx, int nsteps, double *ms_out);
 
 #ifdef __cplusplus
 }

@@ -243,3 +243,26 @@ def test_committed_closure_meets_the_128_vs_512_target_on_a_held_out_seed():
     assert r["closure_status"][0] == 1, r["closure_status"]   # (active, fallback step, max residual)
     assert r["closure_cost_vs_plain"] <= 12.8
     assert r["speedup_vs_fine"] >= 20.0
+
+
+def test_closure_kernel_times_hook():
+    """gabe_closure_kernel_times: four positive per-kernel times of an
+    attached closure, the lattice advanced by the timed steps, and a state
+    error on a lattice without a closure."""
+    from gabe_amd.lattice import GABE_ERR_STATE, GabeError
+    cfg = configs.lambda_phi4_config(N=32, calc_gws=1)
+    lat = host.initialized_lattice(cfg, precision=GABE_FP32)
+    with pytest.raises(GabeError) as e:
+        cm.kernel_times(lat, 2)
+    assert e.value.code == GABE_ERR_STATE
+    cl = cm.Closure(32, cfg.dt, seed=3)
+    cl.set_scales([1e6] * 4, [1e-12] * 2)   # fluctuation-sized inputs, a negligible kick
+    cm.attach(lat, cl, 1e300)
+    n0 = lat.get_scalars()["nstep"]
+    kt = cm.kernel_times(lat, 3)
+    assert lat.get_scalars()["nstep"] == n0 + 3
+    assert set(kt) == set(cm.CLOSURE_KERNELS) and all(v > 0 for v in kt.values()), kt
+    assert cm.status(lat)[0] == 1
+    print({k: f"{v * 1e3:.1f} us" for k, v in kt.items()}, flush=True)
+    lat.close()
+    cl.close()
