@@ -8,7 +8,8 @@ their physical amplitude (no enlarged fluctuations):
 * 512^3 with tensor modes (config 3's lattice), 2 steps, both forms;
 * 64^3 FP64 without tensor modes over 200 steps (the plain CPU workload),
   with the FP32 form beside it;
-* 256^3 FP32 against FP64 over the early window (200 steps);
+* 256^3 FP32 against FP64 over the early window (200 steps) and over config 2's
+  2000 steps with spectra every 200;
 * 512^3 FP32 against FP64 over 400 steps with spectra every 50 (config 3's
   cadence; the full 4000-step run is gabe_amd.precision's);
 * a 64-lattice ensemble of 32^3 ... 128^3 members (bitwise against single
@@ -203,6 +204,26 @@ def test_256_fp32_tracks_fp64_over_the_early_window():
     for k in ("a", "edrho"):
         assert abs(a["scalars"][k][0] - b["scalars"][k][0]) <= 1e-12 * abs(b["scalars"][k][0]), k
     assert a["friedmann"] <= b["friedmann"] * (1 + 1e-6), (a["friedmann"], b["friedmann"])
+
+
+def test_256_fp32_tracks_fp64_over_config2_2000_steps():
+    """Config 2 in full: 256^3 with tensor modes, the FP32 form against the
+    FP64 form from the same device-generated initial state over 2000 steps,
+    spectra of both every 200 steps (gabe_amd.precision): every output's field
+    and GW spectra within 1e-3 per bin, the final field L2 error within 1e-5 of
+    the fluctuation, rho and a within 1e-12, and the FP32 form's Friedmann
+    residual over the run no worse than the FP64 form's (to 1e-6 of it)."""
+    from gabe_amd import precision
+    res = precision.run(256, 2000, 200, log=lambda m: print(m, flush=True))
+    assert len(res["series"]) == 10
+    for row in res["series"]:
+        assert row["field_max_rel"] < 1e-3 and row["gw_max_rel"] < 1e-3, row
+    assert max(res["field_l2_rel"]) < 1e-5, res["field_l2_rel"]
+    assert res["edrho_rel"] < 1e-12 and res["a_rel"] < 1e-12, (res["edrho_rel"], res["a_rel"])
+    assert res["friedmann_max_fp32"] <= res["friedmann_max_fp64"] * (1 + 1e-6)
+    print(f"worst bins: field {res['field_spectra_max_rel']:.2e}, GW {res['gw_spectrum_max_rel']:.2e}; "
+          f"field L2 {max(res['field_l2_rel']):.2e}; Friedmann {res['friedmann_max_fp32']:.8e} / "
+          f"{res['friedmann_max_fp64']:.8e}", flush=True)
 
 
 def test_512_fp32_tracks_fp64_with_spectra_every_50_steps():
