@@ -300,7 +300,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev, hipEvent_t *cl_ev = nullptr)
     if (pipe) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_gw[c->slot], 0));
     CHK(rec(0));
     if (pipe && c->chunk) CHK(chunked_sweep_a(c));
-    else CHK(launch_sweep_ctx(c, SW_A, c->gw));
+    else CHK(launch_sweep_ctx(c, c->gw ? SW_A : c->sweep_a, c->gw));
     CHK(rec(1));
     CHK(rec(2));
     CHK(launch_finalize_ctx(c, SW_A, 0));
@@ -312,7 +312,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev, hipEvent_t *cl_ev = nullptr)
         else CHK(stress_fft(c, false, c->side));
         CHK(gw_modes(c, c->side));
         HIPCHK(hipEventRecord(c->ev_gw[c->slot], c->side));
-        CHK(launch_sweep_ctx(c, SW_B, false));
+        CHK(launch_sweep_ctx(c, c->sweep_b, false));
         c->cur = 1 - c->cur;
         c->slot ^= 1;
         c->shat = c->shat_b[c->slot];
@@ -324,7 +324,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev, hipEvent_t *cl_ev = nullptr)
     if (overlap) {
         HIPCHK(hipEventRecord(c->ev_fin, c->stream));
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fin, 0));
-        CHK(launch_sweep_ctx(c, SW_B, false, c->side));
+        CHK(launch_sweep_ctx(c, c->sweep_b, false, c->side));
         HIPCHK(hipEventRecord(c->ev_side, c->side));
     }
     CHK(rec(4));
@@ -335,7 +335,7 @@ int one_step(gabe_ctx *c, hipEvent_t *ev, hipEvent_t *cl_ev = nullptr)
     CHK(rec(7));
     CHK(rec(8));
     if (overlap) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side, 0));
-    else CHK(launch_sweep_ctx(c, SW_B, false));
+    else CHK(launch_sweep_ctx(c, c->sweep_b, false));
     CHK(rec(9));
     c->cur = 1 - c->cur;
     if (c->pipe) {
@@ -448,6 +448,9 @@ int create_impl(const gabe_params *params, gabe_ctx *c, hipStream_t shared = nul
     c->gwfft = c->colfft && gabe_gwfft_supported(c->N, c->fp32) && !std::getenv("GABE_UNFUSED_GW");
     /* k rows in sweep A: up to GABE_BLOCK sites per row in both forms; 512-site
      * rows (one 512-thread workgroup each) in FP32 */
+    const bool tiled = gabe_sweep_tiled(c->N) && !std::getenv("GABE_UNTILED_SWEEPS");
+    c->sweep_a = tiled && c->fp32 ? SW_A_TILE : SW_A;
+    c->sweep_b = tiled ? SW_B_TILE : SW_B;
     c->rowfft = c->gwfft && c->N >= 16 && (c->N <= GABE_BLOCK || (c->N == 512 && c->fp32)) &&
                 !std::getenv("GABE_LIBRARY_ROWS");
     /* Row length of shat.  Sweep A's rows are N/2 slots, kz = 0 and N/2 packed
@@ -1260,26 +1263,45 @@ struct gabe_ensemble {
     std::vector<gabe_ctx *> members;
     void *d_desc[2] = {nullptr, nullptr};   /* SweepArgs<T>[count] for state parity 0 / 1 */
     FinArgs *d_fin = nullptr;
+    /* the members with LDS-tiled sweeps (FP32, N a multiple of 16) come first
+     * in the descriptor tables and run in launches of their own, so every
+     * member takes the sweeps it would take stepped alone: order[q] is the
+     * member of descriptor q, d_start holds the workgroup starts of the ntile
+     * tiled members, then those of the others */
+    std::vector<int> order;
+    int ntile = 0;
     int *d_start = nullptr;
-    int nblk = 0;
+    int nblk_tile = 0, nblk_rest = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   /* two steps starting at parity 0 / 1 */
 };
 
 namespace {
 
+/* one batched sweep phase (sweep A, or sweep B when b) of every member:
+ * the tiled members in one launch, the others in a second */
+template <typename T>
+void ens_sweep(gabe_ensemble *e, int parity, bool b)
+{
+    const auto *d = (const SweepArgs<T> *)e->d_desc[parity];
+    const int nt = e->ntile, nr = e->count - e->ntile;
+    if (nt) {
+        const gabe_ctx *c = e->members[e->order[0]];
+        launch_sweep_batch<T>(b ? c->sweep_b : c->sweep_a, e->nf, d, e->d_start, nt, e->nblk_tile, e->stream);
+    }
+    if (nr) launch_sweep_batch<T>(b ? SW_B : SW_A, e->nf, d + nt, e->d_start + nt + 1, nr, e->nblk_rest, e->stream);
+}
+
+void ens_phase(gabe_ensemble *e, int parity, bool b)
+{
+    if (e->fp32) ens_sweep<float>(e, parity, b);
+    else ens_sweep<double>(e, parity, b);
+}
+
 int ens_launch_step(gabe_ensemble *e, int parity)
 {
-    if (e->fp32) {
-        const auto *d = (const SweepArgs<float> *)e->d_desc[parity];
-        launch_sweep_batch<float>(SW_A, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-        launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
-        launch_sweep_batch<float>(SW_B, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-    } else {
-        const auto *d = (const SweepArgs<double> *)e->d_desc[parity];
-        launch_sweep_batch<double>(SW_A, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-        launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
-        launch_sweep_batch<double>(SW_B, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-    }
+    ens_phase(e, parity, false);
+    launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
+    ens_phase(e, parity, true);
     HIPCHK(hipGetLastError());
     return GABE_OK;
 }
@@ -1289,11 +1311,11 @@ int ens_upload_desc(gabe_ensemble *e)
 {
     for (int p = 0; p < 2; p++) {
         std::vector<SweepArgs<T>> h(e->count);
-        for (int m = 0; m < e->count; m++) {
-            gabe_ctx *c = e->members[m];
+        for (int q = 0; q < e->count; q++) {
+            gabe_ctx *c = e->members[e->order[q]];
             const int keep = c->cur;
             c->cur = p;
-            h[m] = sweep_args<T>(c);
+            h[q] = sweep_args<T>(c);
             c->cur = keep;
         }
         HIPCHK(hipMalloc(&e->d_desc[p], sizeof(SweepArgs<T>) * e->count));
@@ -1324,14 +1346,11 @@ int ens_create(const gabe_params *params, int count, gabe_ensemble *e)
     e->nf = params[0].nflds;
     e->fp32 = params[0].precision == GABE_FP32;
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    std::vector<int> start(count + 1, 0);
     std::vector<FinArgs> fin(count);
     for (int m = 0; m < count; m++) {
         gabe_ctx *c = new gabe_ctx;
         e->members.push_back(c);
         CHK(create_impl(&params[m], c, e->stream));
-        const int nb = (int)(c->grid.x * c->grid.y);
-        start[m + 1] = start[m] + (nb + GABE_NXCD - 1) / GABE_NXCD * GABE_NXCD;
         FinArgs &F = fin[m];
         F = FinArgs{};
         F.partials = c->partials;
@@ -1347,9 +1366,27 @@ int ens_create(const gabe_params *params, int count, gabe_ensemble *e)
         F.hist = c->hist;
         F.hist_cap = c->hist_cap;
     }
-    e->nblk = start[count];
-    HIPCHK(hipMalloc(&e->d_start, sizeof(int) * (count + 1)));
-    HIPCHK(hipMemcpy(e->d_start, start.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice));
+    for (int pass = 0; pass < 2; pass++)
+        for (int m = 0; m < count; m++)
+            if ((e->members[m]->sweep_b == SW_B_TILE) == (pass == 0)) e->order.push_back(m);
+    e->ntile = 0;
+    for (gabe_ctx *c : e->members) e->ntile += c->sweep_b == SW_B_TILE;
+    /* workgroup starts of each launch, padded to multiples of the XCD count */
+    std::vector<int> start;
+    for (int g = 0; g < 2; g++) {
+        const int q0 = g == 0 ? 0 : e->ntile, q1 = g == 0 ? e->ntile : count;
+        int s0 = 0;
+        start.push_back(0);
+        for (int q = q0; q < q1; q++) {
+            const gabe_ctx *c = e->members[e->order[q]];
+            const int nb = (int)(c->grid.x * c->grid.y);
+            s0 += (nb + GABE_NXCD - 1) / GABE_NXCD * GABE_NXCD;
+            start.push_back(s0);
+        }
+        (g == 0 ? e->nblk_tile : e->nblk_rest) = s0;
+    }
+    HIPCHK(hipMalloc(&e->d_start, sizeof(int) * start.size()));
+    HIPCHK(hipMemcpy(e->d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&e->d_fin, sizeof(FinArgs) * count));
     HIPCHK(hipMemcpy(e->d_fin, fin.data(), sizeof(FinArgs) * count, hipMemcpyHostToDevice));
     if (e->fp32) CHK(ens_upload_desc<float>(e));
@@ -1555,21 +1592,11 @@ int gabe_ensemble_kernel_times(gabe_ensemble *e, int nsteps, double *ms_out)
     for (int n = 0; n < nsteps && rc == GABE_OK; n++, p = 1 - p) {
         hipEvent_t *E = ev.data() + 4 * n;
         (void)hipEventRecord(E[0], e->stream);
-        if (e->fp32) {
-            const auto *d = (const SweepArgs<float> *)e->d_desc[p];
-            launch_sweep_batch<float>(SW_A, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-            (void)hipEventRecord(E[1], e->stream);
-            launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
-            (void)hipEventRecord(E[2], e->stream);
-            launch_sweep_batch<float>(SW_B, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-        } else {
-            const auto *d = (const SweepArgs<double> *)e->d_desc[p];
-            launch_sweep_batch<double>(SW_A, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-            (void)hipEventRecord(E[1], e->stream);
-            launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
-            (void)hipEventRecord(E[2], e->stream);
-            launch_sweep_batch<double>(SW_B, e->nf, d, e->d_start, e->count, e->nblk, e->stream);
-        }
+        ens_phase(e, p, false);
+        (void)hipEventRecord(E[1], e->stream);
+        launch_finalize_batch(e->d_fin, e->nf, e->count, e->stream);
+        (void)hipEventRecord(E[2], e->stream);
+        ens_phase(e, p, true);
         (void)hipEventRecord(E[3], e->stream);
         if (hipGetLastError() != hipSuccess) rc = fail(GABE_ERR_HIP, "batched launch failed");
     }

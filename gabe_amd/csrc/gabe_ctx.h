@@ -102,6 +102,8 @@ struct gabe_ctx {
     bool colfft = false;         /* k-rows by the library, j and i columns by gabe_fft.hip */
     bool gwfft = false;          /* i columns fused with the tensor-mode update */
     bool rowfft = false;         /* k rows transformed inside sweep A */
+    int sweep_a = 1, sweep_b = 2; /* sweep variants: SW_A and SW_B, or SW_A_TILE (without tensor
+                                  * modes) and SW_B_TILE with LDS-staged tiles */
     long kpad = 0;               /* row stride of shat in complex elements (>= N/2+1) */
     int fft_cw = 16;             /* column tile width of the j/i transforms */
     int gw_tile = 1;             /* tile variant of the fused i-transform + mode kernel */

@@ -56,7 +56,27 @@ struct StepScalars {
 __host__ __device__ constexpr int gabe_nsums(int nf) { return 2 + 7 * nf; }
 #define GABE_MAXSUMS (2 + 7 * GABE_MAXF)
 
-enum SweepMode { SW_ENERGY = 0, SW_A = 1, SW_B = 2 };
+/* SW_B_TILE, SW_A_TILE: sweep B, and sweep A without tensor modes, with
+ * LDS-staged plane tiles, the callers' choice where they apply: N a multiple
+ * of 16; sweep A in FP32 only (in FP64 its tiled form needs 182 VGPRs, two
+ * waves per SIMD, against three for the per-thread one) */
+enum SweepMode { SW_ENERGY = 0, SW_A = 1, SW_B = 2, SW_B_TILE = 3, SW_A_TILE = 4 };
+inline bool gabe_sweep_tiled(int N) { return N % 16 == 0; }
+
+/* Loads and stores through global-address-space pointers.  A pointer read
+ * from a descriptor table in memory (the batched launches) is generic to the
+ * compiler, which then emits flat instructions: these also count against the
+ * LDS counter, so every LDS wait waits for them as well. */
+template <typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, T v)
+{
+    *(__attribute__((address_space(1))) T *)p = v;
+}
 
 template <typename T>
 struct SweepArgs {

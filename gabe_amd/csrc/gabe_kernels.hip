@@ -272,7 +272,7 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
     const unsigned ojp = (unsigned)((long)(j + 1 == N ? 0 : j + 1) * N + k) * sizeof(T);
     const unsigned ojm = (unsigned)((long)(j == 0 ? N - 1 : j - 1) * N + k) * sizeof(T);
     auto at = [&](const T *arr, int plane_i, unsigned off) -> T {
-        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(arr + (long)plane_i * N2) + off);
+        return gld(reinterpret_cast<const T *>(reinterpret_cast<const char *>(arr + (long)plane_i * N2) + off));
     };
     const int i0 = by * A.ci;
     const int i1 = min(i0 + A.ci, N);
@@ -348,10 +348,10 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
                 const double dF0 = (double)lap0[f] * ia02 - (Vp0 - mvp0[f]) - 3. * H0 * dv0;
                 const double dv1 = dv0 + hdt * dF0;
                 const double dF1 = (double)lap1[f] * ia12 - (Vp1 - mvp1[f]) - 3. * H1 * dv1;
-                *reinterpret_cast<T *>(reinterpret_cast<char *>(A.fo[f] + (long)i * N2) + ocen) =
-                    (T)((double)bf[QC][f] + dt * dv1);
-                *reinterpret_cast<T *>(reinterpret_cast<char *>(A.vo[f] + (long)i * N2) + ocen) =
-                    (T)(dv0 + dt * dF1);
+                gst(reinterpret_cast<T *>(reinterpret_cast<char *>(A.fo[f] + (long)i * N2) + ocen),
+                    (T)((double)bf[QC][f] + dt * dv1));
+                gst(reinterpret_cast<T *>(reinterpret_cast<char *>(A.vo[f] + (long)i * N2) + ocen),
+                    (T)(dv0 + dt * dF1));
             }
         }
     };
@@ -367,12 +367,228 @@ __device__ __forceinline__ void sweep_b_ahead(const SweepArgs<T> &A, int hw, int
     if (i + 2 < i1) plane(plane_role<2>{}, i + 2);
 }
 
+/* Sweeps with every operand from LDS (sweep B, and sweep A without tensor
+ * modes).  A workgroup owns a TK x TJ tile of (k, j) columns (TK TJ =
+ * GABE_BLOCK, one column per thread; TK = 64, 32 or 16, the largest that
+ * divides N) and marches down its planes of i.  Each plane's tile with its
+ * four one-site edges, (TJ + 2)(TK + 2) slots less the corners, sits in a
+ * ring of four plane slots (planes i - 1, i, i + 1 read while i + 2 is
+ * written); planes i + 2 and i + 3 are in flight in registers, two loads per
+ * thread and array: its own site and one edge site (threads past the
+ * 2 TK + 2 TJ edge sites load their own site again).  So the only global
+ * loads a plane waits for were issued two planes earlier, and the lateral
+ * neighbours cost LDS reads instead of L2 round trips.  The arithmetic is the
+ * per-thread sweeps', operand for operand; sweep A's sums are grouped by
+ * tile instead of by row segment, so its partial sums (not its fields) differ
+ * from the per-thread sweep's in the last bits. */
+constexpr int SBT_SLOT = 6 * 66;   /* (TJ + 2)(TK + 2) at TK = 64, the largest of the shapes */
+
+__device__ __forceinline__ int sbt_tk(int N) { return N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 16); }
+
+template <typename T, int NF, int MODE>
+__device__ __forceinline__ void sweep_tile(const SweepArgs<T> &A, int hw, int gx, int gy, int nhw)
+{
+    static_assert(MODE == SW_B_TILE || MODE == SW_A_TILE, "tiled sweeps: B, and A without tensor modes");
+    constexpr bool SB = MODE == SW_B_TILE;
+    constexpr int NA = 2 * NF;   /* arrays: f[0 .. NF), then v[0 .. NF) */
+    constexpr int NS = gabe_nsums(NF);
+    __shared__ T ring[4][NA][SBT_SLOT];
+    const int N = A.N;
+    const long N2 = A.N2;
+    int bx, by;
+    if (!xcd_block(hw, gx, gy, nhw, bx, by)) return;
+    by += A.by0;
+    const int bid = by * gx + bx;
+    const int TK = sbt_tk(N), TJ = GABE_BLOCK / TK, W = TK + 2;
+    const int ntk = N / TK;
+    const int k0 = (bx % ntk) * TK, j0 = (bx / ntk) * TJ;
+    const int t = threadIdx.x, tr = t / TK, tc = t - tr * TK;
+    const unsigned ocen = (unsigned)((j0 + tr) * N + k0 + tc) * sizeof(T);
+    const int lc = (tr + 1) * W + tc + 1;
+    /* the thread's edge site: rows j0 - 1 and j0 + TJ, then columns k0 - 1
+     * and k0 + TK (periodic) */
+    unsigned oh = ocen;
+    int lh = lc;
+    if (t < 2 * TK + 2 * TJ) {
+        int hj, hk;
+        if (t < TK) {
+            hj = j0 - 1; hk = k0 + t; lh = t + 1;
+        } else if (t < 2 * TK) {
+            hj = j0 + TJ; hk = k0 + t - TK; lh = (TJ + 1) * W + t - TK + 1;
+        } else if (t < 2 * TK + TJ) {
+            hj = j0 + t - 2 * TK; hk = k0 - 1; lh = (t - 2 * TK + 1) * W;
+        } else {
+            hj = j0 + t - 2 * TK - TJ; hk = k0 + TK; lh = (t - 2 * TK - TJ + 1) * W + TK + 1;
+        }
+        hj = hj < 0 ? hj + N : (hj >= N ? hj - N : hj);
+        hk = hk < 0 ? hk + N : (hk >= N ? hk - N : hk);
+        oh = (unsigned)(hj * N + hk) * sizeof(T);
+    }
+    auto at = [&](const T *arr, int plane_i, unsigned off) -> T {
+        return gld(reinterpret_cast<const T *>(reinterpret_cast<const char *>(arr + (long)plane_i * N2) + off));
+    };
+    const int i0 = by * A.ci;
+    const int i1 = min(i0 + A.ci, N);
+    /* plane p of the march, p in [i0 - 1, i1]; loads past i1 read plane i1
+     * again (an L2 hit) into a slot that is never read */
+    auto wrapped = [&](int p) { p = min(p, i1); return p < 0 ? p + N : (p >= N ? p - N : p); };
+    auto load = [&](T (&r)[NA][2], int p) {
+        const int pw = wrapped(p);
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            const T *arr = a < NF ? A.f[a] : A.v[a - NF];
+            r[a][0] = at(arr, pw, ocen);
+            r[a][1] = at(arr, pw, oh);
+        }
+    };
+    auto put = [&](const T (&r)[NA][2], int s) {
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            ring[s][a][lc] = r[a][0];
+            ring[s][a][lh] = r[a][1];
+        }
+    };
+
+    const T idx2 = (T)A.inv_dx2;
+    const double dt = A.dt, hdt = 0.5 * dt;
+    const T hdtT = (T)hdt;
+    /* scalars through global pointers, so that the batched launch (whose
+     * pointers come from its descriptor table) reads them as scalars too:
+     * sweep B the frozen step values, sweep A the current state */
+    double ia02, ia12 = 0., H0 = 0., H1 = 0.;
+    double pb0[2] = {0., 0.}, pb1[2] = {0., 0.}, vb0[2] = {0., 0.}, mvp0[2] = {0., 0.}, mvp1[2] = {0., 0.};
+    if constexpr (SB) {
+        const StepScalars *ss = A.ss;
+        const double a0 = gld(&ss->a0), a1 = gld(&ss->a1);
+        ia02 = 1. / (a0 * a0);
+        ia12 = 1. / (a1 * a1);
+        H0 = gld(&ss->adot0) / a0;
+        H1 = gld(&ss->adot1) / a1;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            pb0[f] = gld(&ss->pbar0[f]); pb1[f] = gld(&ss->pbar1[f]);
+            mvp0[f] = gld(&ss->meanVp0[f]); mvp1[f] = gld(&ss->meanVp1[f]);
+        }
+    } else {
+        const DevScalars *sc = A.sc;
+        const double a0 = gld(&sc->a[0]);
+        ia02 = 1. / (a0 * a0);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            pb0[f] = gld(&sc->pbar[f]);
+            vb0[f] = gld(&sc->vbar[f]);
+            pb1[f] = pb0[f] + hdt * vb0[f];
+        }
+    }
+    double acc[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) acc[q] = 0.;
+
+    /* planes i0 - 1 .. i0 + 1 into slots 0 .. 2; plane i0 + 2 in flight */
+    T rs[2][NA][2];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        load(rs[0], i0 - 1 + q);
+        put(rs[0], q);
+    }
+    load(rs[0], i0 + 2);
+    __syncthreads();
+
+    /* plane i reads slots Q, Q + 1, Q + 2 (planes i - 1, i, i + 1, mod 4),
+     * issues plane i + 3 into register set (Q + 1) & 1 and stores plane
+     * i + 2, in flight since plane i - 1 in set Q & 1, into slot Q + 3 */
+    auto plane = [&](auto role, const int i) {
+        constexpr int Q = decltype(role)::value;
+        constexpr int SM = Q % 4, SC = (Q + 1) % 4, SP = (Q + 2) % 4, SN = (Q + 3) % 4;
+        constexpr int RL = (Q + 1) & 1, RW = Q & 1;
+        load(rs[RL], i + 3);
+        T lap0[NF], lap1[NF], fcs[NF], vcs[NF], c1s[NF];
+        double full0[2] = {0., 0.}, full1[2] = {0., 0.};
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const T *rf = ring[SC][f], *rv = ring[SC][NF + f];
+            const T fm = ring[SM][f][lc], fc = rf[lc], fp = ring[SP][f][lc];
+            const T vm = ring[SM][NF + f][lc], vc = rv[lc], vp = ring[SP][NF + f][lc];
+            const T fkp = rf[lc + 1], fkm = rf[lc - 1];
+            const T fjp = rf[lc + W], fjm = rf[lc - W];
+            lap0[f] = (((fm + fp) + (fjm + fjp)) + (fkm + fkp) - (T)6 * fc) * idx2;
+            full0[f] = pb0[f] + (double)fc;
+            const T vkp = rv[lc + 1], vkm = rv[lc - 1];
+            const T vjp = rv[lc + W], vjm = rv[lc - W];
+            const T m1 = midpt(fm, vm, hdtT), p1 = midpt(fp, vp, hdtT);
+            const T jm1 = midpt(fjm, vjm, hdtT), jp1 = midpt(fjp, vjp, hdtT);
+            const T km1 = midpt(fkm, vkm, hdtT), kp1 = midpt(fkp, vkp, hdtT);
+            const T c1 = midpt(fc, vc, hdtT);
+            lap1[f] = (((m1 + p1) + (jm1 + jp1)) + (km1 + kp1) - (T)6 * c1) * idx2;
+            full1[f] = pb1[f] + (double)c1;
+            fcs[f] = fc;
+            vcs[f] = vc;
+            c1s[f] = c1;
+        }
+        double V0, V1, dV0[2], dV1[2];
+        gabe_V_dV(A.model, full0[0], full0[1], V0, dV0[0], dV0[1]);
+        gabe_V_dV(A.model, full1[0], full1[1], V1, dV1[0], dV1[1]);
+        if constexpr (SB) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const double Vp0 = dV0[f];
+                const double Vp1 = dV1[f];
+                const double dv0 = (double)vcs[f];
+                /* F(0) and F(1) of ddfield (g2functions.cpp:193-196) minus the
+                 * background force */
+                const double dF0 = (double)lap0[f] * ia02 - (Vp0 - mvp0[f]) - 3. * H0 * dv0;
+                const double dv1 = dv0 + hdt * dF0;
+                const double dF1 = (double)lap1[f] * ia12 - (Vp1 - mvp1[f]) - 3. * H1 * dv1;
+                gst(reinterpret_cast<T *>(reinterpret_cast<char *>(A.fo[f] + (long)i * N2) + ocen),
+                    (T)((double)fcs[f] + dt * dv1));
+                gst(reinterpret_cast<T *>(reinterpret_cast<char *>(A.vo[f] + (long)i * N2) + ocen),
+                    (T)(dv0 + dt * dF1));
+            }
+        } else {
+            acc[0] += V0;
+            acc[1] += V1;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const double vel0 = vb0[f] + (double)vcs[f];
+                const int b = 2 + 7 * f;
+                const double Vp0 = dV0[f];
+                const double Vp1 = dV1[f];
+                const double X = (double)lap0[f] * ia02 - Vp0;
+                acc[b + 0] += vel0 * vel0;
+                acc[b + 3] += (double)fcs[f] * (double)lap0[f];
+                acc[b + 1] += vel0 * X;
+                acc[b + 2] += X * X;
+                acc[b + 4] += (double)c1s[f] * (double)lap1[f];
+                acc[b + 5] += Vp0;
+                acc[b + 6] += Vp1;
+            }
+        }
+        put(rs[RW], SN);
+        __syncthreads();
+    };
+    int i = i0;
+    for (; i + 3 < i1; i += 4) {
+        plane(plane_role<0>{}, i);
+        plane(plane_role<1>{}, i + 1);
+        plane(plane_role<2>{}, i + 2);
+        plane(plane_role<3>{}, i + 3);
+    }
+    if (i < i1) plane(plane_role<0>{}, i);
+    if (i + 1 < i1) plane(plane_role<1>{}, i + 1);
+    if (i + 2 < i1) plane(plane_role<2>{}, i + 2);
+    if constexpr (!SB) block_store_partials<NS, 1>(acc, A.partials, A.nblk, bid);
+}
+
 template <typename T, int NF, int MODE, bool GW, int ROWN = 0>
 __device__ __forceinline__ void sweep_body(const SweepArgs<T> &A, int hw, int gx, int gy, int nhw,
                                            typename gabe_fft::cplx_t<T>::type *zl = nullptr,
                                            typename gabe_fft::cplx_t<T>::type *zo = nullptr,
                                            typename gabe_fft::cplx_t<T>::type *zt = nullptr)
 {
+    if constexpr (MODE == SW_B_TILE || MODE == SW_A_TILE) {
+        sweep_tile<T, NF, MODE>(A, hw, gx, gy, nhw);
+        return;
+    }
     if constexpr (MODE == SW_B && sizeof(T) == 4) {
         sweep_b_ahead<T, NF>(A, hw, gx, gy, nhw);
         return;
@@ -1181,6 +1397,10 @@ static void launch_sweep_nf(int mode, bool gw, const SweepArgs<T> &a, dim3 grid,
     if (mode == SW_ENERGY) hipLaunchKernelGGL((k_sweep<T, NF, SW_ENERGY, false>), grid, dim3(GABE_BLOCK), 0, s, a);
     else if (mode == SW_A && gw) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, true>), grid, dim3(GABE_BLOCK), 0, s, a);
     else if (mode == SW_A) hipLaunchKernelGGL((k_sweep<T, NF, SW_A, false>), grid, dim3(GABE_BLOCK), 0, s, a);
+    else if (mode == SW_B_TILE)
+        hipLaunchKernelGGL((k_sweep<T, NF, SW_B_TILE, false>), grid, dim3(GABE_BLOCK), 0, s, a);
+    else if (mode == SW_A_TILE && sizeof(T) == 4)
+        hipLaunchKernelGGL((k_sweep<T, NF, SW_A_TILE, false>), grid, dim3(GABE_BLOCK), 0, s, a);
     else hipLaunchKernelGGL((k_sweep<T, NF, SW_B, false>), grid, dim3(GABE_BLOCK), 0, s, a);
 }
 
@@ -1192,6 +1412,10 @@ static void launch_sweep_batch_nf(int mode, const SweepArgs<T> *d, const int *st
         hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_ENERGY, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
     else if (mode == SW_A)
         hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_A, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
+    else if (mode == SW_B_TILE)
+        hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_B_TILE, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
+    else if (mode == SW_A_TILE && sizeof(T) == 4)
+        hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_A_TILE, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
     else hipLaunchKernelGGL((k_sweep_batch<T, NF, SW_B, false>), dim3(nblk), dim3(GABE_BLOCK), 0, s, d, start, count);
 }
 

@@ -25,9 +25,12 @@ def member_cfgs(sizes, seed=3):
     return out
 
 
-@pytest.mark.parametrize("nsteps", [10, 7])
-def test_members_match_single_lattices_bitwise(nsteps):
-    cfgs = member_cfgs([16, 24, 32, 16])
+@pytest.mark.parametrize("nsteps,sizes", [(10, (16, 24, 32, 16)), (7, (16, 24, 32, 16)), (9, (16, 32, 48, 64))])
+def test_members_match_single_lattices_bitwise(nsteps, sizes):
+    """With a 24^3 member the batched sweep B is the per-thread one while the
+    single 16^3 and 32^3 lattices use the LDS-tiled one; with sizes that are
+    all multiples of 16 both sides are tiled."""
+    cfgs = member_cfgs(list(sizes))
     ens = Ensemble([dict(precision=GABE_FP32, **c.lattice_kwargs()) for c in cfgs])
     singles = []
     for c, m in zip(cfgs, ens.members):

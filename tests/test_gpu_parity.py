@@ -340,6 +340,51 @@ def test_rows_transformed_in_sweep_match_library_rows(precision, N, monkeypatch)
     assert rel_err(lf, ll) < tol and rel_err(hf, hl) < tol
 
 
+@pytest.mark.parametrize("precision,N,ppb,gw", [(GABE_FP32, 16, 16, 1), (GABE_FP32, 32, 7, 0), (GABE_FP32, 48, 16, 1),
+                                                (GABE_FP32, 64, 5, 1), (GABE_FP32, 96, 32, 0), (GABE_FP32, 128, 32, 1),
+                                                (GABE_FP32, 256, 32, 1), (GABE_FP64, 32, 7, 1), (GABE_FP64, 64, 32, 0),
+                                                (GABE_FP64, 128, 16, 1)])
+def test_tiled_sweeps_match_untiled(precision, N, ppb, gw, monkeypatch):
+    """Sweeps reading their planes from LDS-staged tiles (64 x 4, 32 x 8 or
+    16 x 16 columns with one-site edges, two planes in flight) against the
+    per-thread sweeps that load every neighbour themselves, for each tile
+    shape and for plane counts per workgroup that leave partial march groups.
+    With tensor modes, or in FP64, only sweep B is tiled and the bits agree;
+    in FP32 without them sweep A is tiled too, its sums are grouped by tile
+    rather than by row, and the energies agree to FP64 rounding, the fields to
+    FP32 rounding."""
+    from gabe_amd import host
+    cfg = configs.lambda_phi4_config(N=N, calc_gws=gw)
+    tiled = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb)
+    monkeypatch.setenv("GABE_UNTILED_SWEEPS", "1")
+    plain = host.initialized_lattice(cfg, precision=precision, planes_per_block=ppb)
+    monkeypatch.delenv("GABE_UNTILED_SWEEPS")
+    bitwise = gw or precision == GABE_FP64
+    st, sp = tiled.get_scalars(), plain.get_scalars()
+    if not bitwise:
+        # the energies of the initial state: one tiled sweep A against one per-thread sweep A
+        for k in ("edkin", "edpot", "edgrad", "edrho"):
+            assert abs(st[k][0] - sp[k][0]) <= 1e-14 * abs(sp["edrho"][0]), k
+    tiled.step(9)
+    plain.step(9)
+    ft, dt = tiled.download_state()
+    fp, dp = plain.download_state()
+    st, sp = tiled.get_scalars(), plain.get_scalars()
+    if bitwise:
+        assert np.array_equal(ft, fp) and np.array_equal(dt, dp)
+        assert st["a"][0] == sp["a"][0]
+    else:
+        assert fluct_err(ft, fp) < 1e-5 and fluct_err(dt, dp) < 1e-5
+        assert abs(st["a"][0] - sp["a"][0]) <= 1e-14 * sp["a"][0]
+        assert abs(st["edrho"][0] - sp["edrho"][0]) <= 1e-12 * abs(sp["edrho"][0])
+    if gw:
+        ht, lt = tiled.download_gw()
+        hp, lp = plain.download_gw()
+        assert np.array_equal(ht, hp) and np.array_equal(lt, lp)
+    tiled.close()
+    plain.close()
+
+
 @pytest.mark.parametrize("ppb", [5, 6, 7])
 def test_plane_counts_per_workgroup_agree(ppb):
     """Sweep workgroups marching 5, 6 or 7 planes give the fields and tensor
