@@ -232,14 +232,27 @@ def run_closure(steps, warmup):
     # useful multiply-adds on the matrix cores, 2 x c_in x c_out x 27 flop per site and convolution
     n3 = cfg.N ** 3
     conv_flops = {"first": 2.0 * 4 * 16 * 27 * n3, "hidden": 2.0 * 16 * 16 * 27 * n3, "output": 2.0 * 16 * 2 * 27 * n3}
+    # algorithmic HBM bytes per site: the first layer reads the four FP32 lattice arrays and writes 16 BF16
+    # channels, the hidden one reads and writes 16 BF16 channels, the output one reads 16 and writes 2 FP32
+    conv_bytes = {"first": 48.0 * n3, "hidden": 64.0 * n3, "output": 40.0 * n3}
     kt = out["kernels_ms"]
     dom = max(conv_flops, key=lambda k: kt[k])
-    achieved = conv_flops[dom] / (kt[dom] * 1e-3) / 1e12
-    out["roofline"] = {"bound": "mfma", "kernel": f"{dom} convolution", "dtype": "bf16 (fp32 accumulate)",
-                       "flop_per_launch": conv_flops[dom], "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
-                       "frac": achieved / 2500.0,
-                       "note": "the longest convolution's useful flops over its mean HIP-event duration in "
-                               "steps launched one kernel after another"}
+    sec = kt[dom] * 1e-3
+    tf = conv_flops[dom] / sec / 1e12
+    gbs = conv_bytes[dom] / sec / 1e9
+    # the bound is the roof that allows the longer time: 216 flop/B for the hidden convolution, below the
+    # BF16 ridge of 2.5 PFLOP/s over 8 TB/s = 312 flop/B, so HBM
+    hbm_bound = conv_bytes[dom] / 8e12 >= conv_flops[dom] / 2.5e15
+    out["roofline"] = {"bound": "hbm" if hbm_bound else "mfma", "kernel": f"{dom} convolution",
+                       "dtype": "bf16 (fp32 accumulate)", "alg_bytes_per_launch": conv_bytes[dom],
+                       "flop_per_launch": conv_flops[dom],
+                       "achieved": gbs if hbm_bound else tf, "peak": 8000.0 if hbm_bound else 2500.0,
+                       "unit": "GB/s" if hbm_bound else "TFLOP/s",
+                       "frac": gbs / 8000.0 if hbm_bound else tf / 2500.0,
+                       "mfma": {"achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0},
+                       "hbm": {"achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0},
+                       "note": "the longest convolution over its mean HIP-event duration in steps launched one "
+                               "kernel after another; bound = the roof allowing the longer time"}
     # the whole network per step, from the step times (includes the kick and the side stream's share of the GPU)
     flops = sum(conv_flops.values())
     t = (out["closure_ms_per_step"] - out["plain_ms_per_step"]) * 1e-3
