@@ -436,49 +436,118 @@ __device__ __forceinline__ float2 lap2(const float *u, long s, int N, long N2)
 }
 
 /* The correction after a step, v_c += dt s_out[c] (lap w)_c / a^2 with
- * w = lap u, in one pass: a workgroup owns a KI x KJ x KK block of sites,
- * computes w on the block and its 1-site halo into LDS (all of the u reads
- * of the block in flight at once), then corrects the block from LDS.  Both
- * Laplacians are lap2's arithmetic, site by site.  Blocks go to the XCDs in
- * contiguous ranges, so the halo reads of neighbouring blocks meet in one L2. */
-constexpr int KI = 4, KJ = 8, KK = 32, HJ = KJ + 2, HK = KK + 2, HP = HJ * HK, KH = (KI + 2) * HP;
+ * w = lap u, in one march: a workgroup owns a TK x TJ tile of (k, j) columns
+ * (64 x 4 or 32 x 8, one column per thread) and D planes of i.  Planes of u
+ * with a two-site edge, (TJ + 4)(TK + 4) sites, sit in a ring of four LDS
+ * slots and planes of w with a one-site edge in a ring of three; plane i + 3
+ * of u is in flight in registers while w(i + 1) is computed and plane i
+ * corrected, and plane i's velocities are loaded before w(i + 1) is.  Both
+ * Laplacians are lap2's arithmetic, operand for operand.  Workgroups go to
+ * the XCDs in contiguous ranges, so the edge reads of neighbouring tiles meet
+ * in one L2. */
+constexpr int KU = 8 * 68, KW = 6 * 66;   /* slot sizes at 64 x 4, the larger of the two tile shapes */
 
-__device__ __forceinline__ float2 lap2_tile(const float2 *w, int h)
-{
-    const float2 c = w[h];
-    float2 sum = w[h + HP];
-    for (float2 v : {w[h - HP], w[h + HK], w[h - HK], w[h + 1], w[h - 1]}) {
-        sum.x += v.x;
-        sum.y += v.y;
-    }
-    return make_float2(sum.x - 6.f * c.x, sum.y - 6.f * c.y);
-}
+__host__ __device__ inline int kick_tk(int N) { return N % 64 == 0 ? 64 : 32; }
 
-__global__ __launch_bounds__(256) void k_lap2_kick(const float *u, int N, float *v0, float *v1, const DevScalars *sc,
-                                                   float dts0, float dts1, const int *gate)
+__global__ __launch_bounds__(256) void k_lap2_kick(const float *u, int N, int D, float *v0, float *v1,
+                                                   const DevScalars *sc, float dts0, float dts1, const int *gate)
 {
     if (gate && gate[0] == 0) return;
-    __shared__ float2 w[KH];
-    const long N2 = (long)N * N;
-    const int tid = threadIdx.x, ntk = N / KK, ntj = N / KJ, wg = logical_wg();
-    const int k0 = (wg % ntk) * KK, j0 = ((wg / ntk) % ntj) * KJ, i0 = (wg / (ntk * ntj)) * KI;
-#pragma unroll 4
-    for (int h = tid; h < KH; h += 256) {
-        const int hi = h / HP, hr = h - hi * HP;
-        const int i = wrap(i0 + hi - 1, N), j = wrap(j0 + hr / HK - 1, N), k = wrap(k0 + hr % HK - 1, N);
-        w[h] = lap2(u, ((long)i * N + j) * N + k, N, N2);
+    __shared__ float2 us[4][KU];
+    __shared__ float2 ws[3][KW];
+    const int TK = kick_tk(N), TJ = 256 / TK, UW = TK + 4, WW = TK + 2;
+    const int nu = (TJ + 4) * UW, nw = (TJ + 2) * WW;
+    const int tid = threadIdx.x, ntk = N / TK, ntj = N / TJ, wg = logical_wg();
+    const int k0 = (wg % ntk) * TK, j0 = ((wg / ntk) % ntj) * TJ, i0 = (wg / (ntk * ntj)) * D;
+    const int i1 = i0 + D, N2 = N * N;
+    const float2 *u2 = reinterpret_cast<const float2 *>(u);
+    /* the thread's u sites of a plane: slot index tid + 256 m, m < 3 */
+    int uoff[3];
+    bool uval[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const int h = tid + 256 * m;
+        uval[m] = h < nu;
+        const int hh = uval[m] ? h : 0, r = hh / UW, c = hh - r * UW;
+        uoff[m] = wrap(j0 + r - 2, N) * N + wrap(k0 + c - 2, N);
     }
-    __syncthreads();
+    /* plane p of the march, p in [i0 - 2, i1 + 1]; loads past i1 + 1 read
+     * plane i1 + 1 again (an L2 hit) into a slot that is never read */
+    auto uload = [&](float2 (&r)[3], int p) {
+        const int pw = wrap(min(p, i1 + 1), N);
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            if (uval[m]) r[m] = u2[pw * N2 + uoff[m]];
+    };
+    auto uput = [&](const float2 (&r)[3], int slot) {
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            if (uval[m]) us[slot][tid + 256 * m] = r[m];
+    };
+    auto su = [&](int p) { return (p - i0 + 2) & 3; };
+    auto sw = [&](int p) { return (p - i0 + 1) % 3; };
+    /* w at plane p from u planes p - 1, p, p + 1 */
+    auto wcomp = [&](int p) {
+        const float2 *um = us[su(p - 1)], *uc = us[su(p)], *up = us[su(p + 1)];
+        float2 *w = ws[sw(p)];
+        for (int h = tid; h < nw; h += 256) {
+            const int r = h / WW, c = h - r * WW, q = (r + 1) * UW + c + 1;
+            const float2 cc = uc[q];
+            float2 sum = up[q];
+            for (float2 v : {um[q], uc[q + UW], uc[q - UW], uc[q + 1], uc[q - 1]}) {
+                sum.x += v.x;
+                sum.y += v.y;
+            }
+            w[h] = make_float2(sum.x - 6.f * cc.x, sum.y - 6.f * cc.y);
+        }
+    };
     const double a = sc->a[0];
     const float ia2 = (float)(1. / (a * a));
-    const int tj = tid / KK, tk = tid % KK;
+    const int tr = tid / TK, tc = tid - tr * TK, rc = (tr + 1) * WW + tc + 1;
+    const int soff = (j0 + tr) * N + k0 + tc;
+
+    float2 r[3];
 #pragma unroll
-    for (int ti = 0; ti < KI; ti++) {
-        const float2 y = lap2_tile(w, (ti + 1) * HP + (tj + 1) * HK + tk + 1);
-        const long s = ((long)(i0 + ti) * N + j0 + tj) * N + k0 + tk;
-        v0[s] += dts0 * ia2 * y.x;
-        v1[s] += dts1 * ia2 * y.y;
+    for (int q = 0; q < 4; q++) {
+        uload(r, i0 - 2 + q);
+        uput(r, q);
     }
+    __syncthreads();
+    wcomp(i0 - 1);
+    wcomp(i0);
+    __syncthreads();
+    uload(r, i0 + 2);
+    uput(r, su(i0 + 2));
+    __syncthreads();
+    for (int i = i0; i < i1; i++) {
+        const int s = i * N2 + soff;
+        const float va = v0[s], vb = v1[s];
+        uload(r, i + 3);
+        wcomp(i + 1);
+        __syncthreads();
+        const float2 *wm = ws[sw(i - 1)], *wc = ws[sw(i)], *wp = ws[sw(i + 1)];
+        const float2 cc = wc[rc];
+        float2 sum = wp[rc];
+        for (float2 v : {wm[rc], wc[rc + WW], wc[rc - WW], wc[rc + 1], wc[rc - 1]}) {
+            sum.x += v.x;
+            sum.y += v.y;
+        }
+        const float2 y = make_float2(sum.x - 6.f * cc.x, sum.y - 6.f * cc.y);
+        v0[s] = va + dts0 * ia2 * y.x;
+        v1[s] = vb + dts1 * ia2 * y.y;
+        uput(r, su(i + 3));
+        __syncthreads();
+    }
+}
+
+/* planes per kick workgroup: the most of 16, 8, 4 that still gives 512
+ * workgroups */
+int kick_depth(int N)
+{
+    const int TK = kick_tk(N), tiles = (N / TK) * (N / (256 / TK));
+    for (int d : {16, 8, 4})
+        if (N % d == 0 && tiles * (N / d) >= 512) return d;
+    return 4;
 }
 
 /* loss terms (w - t)^2 (block partials) and dL/dw = (w - t) / N^3 for
@@ -1179,10 +1248,10 @@ int closure_after_step(gabe_ctx *c, hipEvent_t *ev)
     CHK(rec(5));
     float *vw = (float *)c->vbuf[c->cur];
     const int N = c->N;
-    static_assert(KJ * KK == 256, "one site of each plane of a block per thread");
     CHK(rec(6));
-    hipLaunchKernelGGL(k_lap2_kick, dim3((N / KK) * (N / KJ) * (N / KI)), dim3(256), 0, c->stream, R->u, N, vw,
-                       vw + c->N3, c->sc, R->dt_s[0], R->dt_s[1], R->gate);
+    const int TK = kick_tk(N), D = kick_depth(N);
+    hipLaunchKernelGGL(k_lap2_kick, dim3((N / TK) * (N / (256 / TK)) * (N / D)), dim3(256), 0, c->stream, R->u, N, D,
+                       vw, vw + c->N3, c->sc, R->dt_s[0], R->dt_s[1], R->gate);
     HIPCHK(hipGetLastError());
     CHK(rec(7));
     return GABE_OK;
