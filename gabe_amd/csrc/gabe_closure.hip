@@ -540,13 +540,14 @@ __global__ __launch_bounds__(256) void k_lap2_kick(const float *u, int N, int D,
     }
 }
 
-/* planes per kick workgroup: the most of 16, 8, 4 that still gives 512
- * workgroups */
+/* planes per kick workgroup: the most of 16, 8, 4 that still gives 1024
+ * workgroups (128^3: 8 planes, 20.0-20.7 us against 23.1 at 16 and 22.5 at
+ * 4, whose ring fills cost more than its four workgroups per CU gain) */
 int kick_depth(int N)
 {
     const int TK = kick_tk(N), tiles = (N / TK) * (N / (256 / TK));
     for (int d : {16, 8, 4})
-        if (N % d == 0 && tiles * (N / d) >= 512) return d;
+        if (N % d == 0 && tiles * (N / d) >= 1024) return d;
     return 4;
 }
 
