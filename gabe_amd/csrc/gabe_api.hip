@@ -1263,7 +1263,7 @@ struct gabe_ensemble {
     std::vector<gabe_ctx *> members;
     void *d_desc[2] = {nullptr, nullptr};   /* SweepArgs<T>[count] for state parity 0 / 1 */
     FinArgs *d_fin = nullptr;
-    /* the members with LDS-tiled sweeps (FP32, N a multiple of 16) come first
+    /* the members with LDS-tiled sweeps (N a multiple of 16) come first
      * in the descriptor tables and run in launches of their own, so every
      * member takes the sweeps it would take stepped alone: order[q] is the
      * member of descriptor q, d_start holds the workgroup starts of the ntile
