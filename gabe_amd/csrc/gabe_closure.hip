@@ -1174,6 +1174,9 @@ int restrict_array(gabe_ctx *f, gabe_ctx *c, void *src, void *dst, bool inverse_
                            (const double *)f->rx_buf, f->N, (double *)c->rx_buf, c->N, scale, inv_lap);
     HIPCHK(hipGetLastError());
     CHK(run_plan(c, c->fft_rx, c->rx_buf, dst));
+    /* the coarse stream reads f->rx_buf, which the fine stream's transform of
+     * the next array rewrites: nothing orders the two streams, so wait here */
+    HIPCHK(hipStreamSynchronize(c->stream));
     return GABE_OK;
 }
 
@@ -1470,6 +1473,9 @@ int gabe_closure_sample(gabe_closure *cl, gabe_ctx *fine, gabe_ctx *coarse, int 
                        (float)(1. / cl->s_in[0]), (float)(1. / cl->s_in[1]), (float)(1. / cl->s_in[2]),
                        (float)(1. / cl->s_in[3]), N3, x, (const int *)nullptr);
     HIPCHK(hipGetLastError());
+    /* k_make_x reads the coarse state on the closure's stream; the coarse
+     * stream rewrites it below (gabe_restrict after the fine steps) */
+    HIPCHK(hipStreamSynchronize(cl->stream));
     /* target: restriction of the fine lattice's Laplacian minus the coarse
      * lattice's Laplacian of the restricted field */
     hipLaunchKernelGGL(k_lap_field, dim3(nblocks(N3f, 256)), dim3(256), 0, fine->stream,
