@@ -3,7 +3,13 @@ the pipelined step (sweep A in chunks with the j-transform on a stream of its
 own, the tensor modes on the side stream, sweep B on LDS-staged tiles), the
 batched ensemble, and the restriction of a fine lattice followed by steps
 with a learned closure attached.  A missing wait between streams shows up
-here as results that change from run to run."""
+here as results that change from run to run.  A fresh process repeats the
+same bits as well."""
+import hashlib
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -80,3 +86,41 @@ def test_restriction_and_closure_steps_repeat_bitwise():
 
     _twice(run)
     cl.close()
+
+
+def _digest(*arrs):
+    m = hashlib.sha256()
+    for a in arrs:
+        m.update(np.ascontiguousarray(a).tobytes())
+    return m.hexdigest()
+
+
+def _hashes():
+    """Digests of a 128^3 step run with tensor modes and of a 128^3 run with
+    the trained closure attached, 20 steps each."""
+    out = []
+    cfg = configs.lambda_phi4_config(N=128, calc_gws=1)
+    lat = host.initialized_lattice(cfg, precision=GABE_FP32, planes_per_block=32, transform_tile=16, mode_tile=1,
+                                   stress_chunk=64)
+    lat.step(20)
+    out.append(_digest(*lat.download_state(), *lat.download_gw(), np.array([lat.get_scalars()["a"][0]])))
+    lat.close()
+    cl = cm.Closure.from_file(os.path.join(os.path.dirname(cm.__file__), "closure_128.npz"))
+    lat = host.initialized_lattice(cfg, precision=GABE_FP32, planes_per_block=32, transform_tile=16, mode_tile=1,
+                                   stress_chunk=-1)
+    cm.attach(lat, cl, 1e-3)
+    lat.step(20)
+    assert cm.status(lat)[0] == 1
+    out.append(_digest(*lat.download_state()))
+    lat.close()
+    cl.close()
+    return out
+
+
+def test_fresh_process_repeats_bitwise():
+    here = os.path.dirname(os.path.abspath(__file__))
+    code = ("import sys; sys.path[:0] = [%r, %r]; import test_determinism as t; print(' '.join(t._hashes()))"
+            % (os.path.dirname(here), here))
+    child = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert child.returncode == 0, child.stderr[-2000:]
+    assert child.stdout.split() == _hashes()
